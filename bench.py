@@ -1,0 +1,168 @@
+"""Benchmark: Mqueries/s of 3D closest-point queries on one MI355X.
+
+Workload (the headline): displaced icosphere, 7 subdivisions (163,842 vertices, 327,680
+triangles), 1,048,576 query points uniform in [-1.5, 1.5]^3, all resident in HBM before timing.
+A step is one Mesh.closest_point_bvh call over the whole batch.  The CPU baseline is the FP32
+restatement of the reference traversal (oracle/gq_oracle.c) on the host cores, over a bounded
+sample of the same queries.
+
+    python bench.py [--steps K] [--warmup W] [--method engine|engine_fp32|plain]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+N_QUERIES = 1 << 20
+SUBDIV = 7
+
+# Reference float32 record sizes (bytes): node 36 B (BVH3D, 9 floats), triangle 40 B (10 floats),
+# query point 12 B, ClosestPointRecord3D 41 B (valid 1 + p 12 + n 12 + uv 8 + d 4 + prim_id 4).
+S_NODE, S_PRIM, S_IN, S_OUT = 36, 40, 12, 41
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_query(scene, x_sample):
+    """B_q = S_in + S_out + S_node * (popped nodes + child-box tests) + S_prim * (leaf tests + 1),
+    counted by the CPU restatement of the reference traversal on a sample of the queries."""
+    from oracle import oracle as O
+    s = scene.snch
+    r = O.closest_point(3, s.bvh_node_data(), s.primitive_data(), x_sample)
+    visits = float(r["visits"].mean())
+    child = float(r["child_tests"].mean())
+    leaf = float(r["leaf_tests"].mean())
+    bq = S_IN + S_OUT + S_NODE * (visits + child) + S_PRIM * (leaf + 1)
+    return bq, {"visits": visits, "child_tests": child, "leaf_tests": leaf}
+
+
+def cpu_baseline(scene, x, budget_s=10.0):
+    """Oracle FP32 traversal on all host threads over a bounded prefix of the batch."""
+    from oracle import oracle as O
+    s = scene.snch
+    nodes, prims = s.bvh_node_data(), s.primitive_data()
+    n = 16384
+    t0 = time.perf_counter()
+    O.closest_point(3, nodes, prims, x[:n])
+    dt = time.perf_counter() - t0
+    n = int(min(len(x), max(n, n * budget_s / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    O.closest_point(3, nodes, prims, x[:n])
+    dt = time.perf_counter() - t0
+    return {"value": n / dt / 1e6, "unit": "Mqueries/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"first {n} of the {len(x)} queries, FP32 restatement of BVH3D.closest_point, "
+                      f"OpenMP over {O.num_threads()} host threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--method", default="engine", choices=["engine", "engine_fp32", "plain"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; queries shard by batch with no "
+                                   "exchange, so multi-GPU numbers are not measured here"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gquery_amd import _lib, synth
+    from gquery_amd.shapes.mesh import Mesh
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    V, F = synth.displaced_icosphere(SUBDIV)
+    mesh = Mesh(V, F, device="cuda:0")
+    log(f"scene built in {time.perf_counter() - t0:.1f}s: {F.shape[0]} triangles")
+    x_host = synth.uniform_points(N_QUERIES, 3)
+    x = torch.from_numpy(x_host).cuda()
+    torch.cuda.synchronize()
+
+    method = args.method
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        mesh.closest_point_bvh(x, method=method)
+    torch.cuda.synchronize()
+    _lib.profile_reset()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        rec = mesh.closest_point_bvh(x, method=method)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = ev0.elapsed_time(ev1)
+    kern_ms, kern_launches, kern_name = _lib.profile_read()
+    ms_per_step = ms_total / args.steps
+    value = N_QUERIES / (ms_per_step * 1e-3) / 1e6
+    log(f"{method}: {ms_per_step:.3f} ms/step ({wall / args.steps * 1e3:.3f} ms wall), {value:.1f} Mq/s; "
+        f"traversal kernel {kern_name} {kern_ms / max(kern_launches, 1):.3f} ms/launch")
+
+    # roofline of the dominant (traversal) kernel
+    rng = np.random.default_rng(0)
+    sample = x_host[rng.choice(N_QUERIES, 20000, replace=False)]
+    bq, counts = algorithmic_bytes_per_query(mesh, sample)
+    kern_avg_s = kern_ms / max(kern_launches, 1) * 1e-3
+    achieved = bq * N_QUERIES / kern_avg_s / 1e9 if kern_avg_s > 0 else None
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_cp3d_s7_1M_{method}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+            "kernel": kern_name, "kernel_ms": kern_avg_s * 1e3, "bytes_per_query": bq, "work_per_query": counts}
+
+    # sanity: a sample of the timed results against the oracle
+    from oracle import oracle as O
+    idx = rng.choice(N_QUERIES, 2000, replace=False)
+    ref = O.closest_point(3, mesh.snch.bvh_node_data(), mesh.snch.primitive_data(), x_host[idx])
+    got_d = rec.d.cpu().numpy()[idx]
+    max_err = float(np.max(np.abs(got_d - ref["d"]) / np.maximum(ref["d"], 1.0)))
+    log(f"sample check vs oracle: max rel err d = {max_err:.2e}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mesh, x_host)
+        log(f"cpu baseline: {cpu['value']:.3f} Mq/s ({cpu['sample']})")
+
+    out = {
+        "metric": "Mqueries/s (3D closest-point, 327k tris, 1M q)",
+        "value": value,
+        "unit": "Mqueries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded displaced icosphere + uniform queries",
+        "config": {"workload": "closest_point_bvh, displaced icosphere s=7 (327,680 triangles), "
+                               "1,048,576 uniform queries in [-1.5,1.5]^3",
+                   "method": method, "global_batch": N_QUERIES, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "max_rel_err_vs_oracle": max_err,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,353 @@
+// C-ABI layer: builders, device scenes and query dispatch (declarations in include/gquery_amd.h).
+#include <cstring>
+#include <mutex>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "builder.h"
+#include "gq_internal.h"
+#include "wide.h"
+
+struct gq_structure {
+    gqa::HostStructure h;
+};
+
+struct gq_scene {
+    int dim = 0;
+    bool has_snch = false;
+    int device = 0;
+    int max_depth = 0;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+    gqa::PlainView plain{};
+    gqa::WideView wide{};
+    gqa::EngineWorkspace ws;
+};
+
+namespace gqa {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+// ---- traversal-kernel timing ----
+namespace {
+struct Prof {
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::string name;
+    hipEvent_t pending = nullptr;
+};
+Prof &prof() {
+    static Prof p;
+    return p;
+}
+} // namespace
+
+void prof_begin(hipStream_t st) {
+    Prof &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    if (!p.on) return;
+    if (hipEventCreate(&p.pending) != hipSuccess) { p.pending = nullptr; return; }
+    hipEventRecord(p.pending, st);
+}
+
+void prof_end(hipStream_t st, const char *kernel) {
+    Prof &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    if (!p.on || !p.pending) return;
+    hipEvent_t e1;
+    if (hipEventCreate(&e1) != hipSuccess) return;
+    hipEventRecord(e1, st);
+    p.ev.emplace_back(p.pending, e1);
+    p.pending = nullptr;
+    p.name = kernel;
+}
+
+static int fail(int code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? GQ_ERR_OOM : GQ_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+static int upload(gq_scene *s, const T *host, size_t count, const T **dev) {
+    *dev = nullptr;
+    if (count == 0) return GQ_OK;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    s->allocs.push_back(p);
+    s->bytes += count * sizeof(T);
+    e = hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+    *dev = (const T *)p;
+    return GQ_OK;
+}
+
+} // namespace gqa
+
+using namespace gqa;
+
+extern "C" {
+
+int gq_version(void) { return 100; }
+
+const char *gq_last_error(void) { return g_last_error.c_str(); }
+
+static int build_common(int dim, const float *V, int64_t nv, const int32_t *I, int64_t np,
+                        int32_t max_prims, bool snch, gq_structure **out) {
+    if (!out) return fail(GQ_ERR_INVALID, "out is null");
+    *out = nullptr;
+    std::unique_ptr<gq_structure> s(new (std::nothrow) gq_structure());
+    if (!s) return fail(GQ_ERR_OOM, "host allocation failed");
+    std::string err;
+    try {
+        if (!build_structure(dim, V, nv, I, np, max_prims, snch, s->h, err))
+            return fail(GQ_ERR_INVALID, err);
+    } catch (const std::bad_alloc &) {
+        return fail(GQ_ERR_OOM, "host allocation failed during build");
+    }
+    *out = s.release();
+    return GQ_OK;
+}
+
+int gq_build_bvh(int dim, const float *vertices, int64_t n_vertices, const int32_t *indices,
+                 int64_t n_prims, int32_t max_prims_in_node, gq_structure **out) {
+    return build_common(dim, vertices, n_vertices, indices, n_prims, max_prims_in_node, false, out);
+}
+
+int gq_build_snch(int dim, const float *vertices, int64_t n_vertices, const int32_t *indices,
+                  int64_t n_prims, int32_t max_prims_in_node, gq_structure **out) {
+    return build_common(dim, vertices, n_vertices, indices, n_prims, max_prims_in_node, true, out);
+}
+
+void gq_structure_free(gq_structure *s) { delete s; }
+
+int gq_structure_info(const gq_structure *s, int64_t info[8]) {
+    if (!s || !info) return fail(GQ_ERR_INVALID, "null argument");
+    const HostStructure &h = s->h;
+    info[0] = h.dim;
+    info[1] = h.is_snch ? 1 : 0;
+    info[2] = h.n_nodes;
+    info[3] = h.n_prims;
+    info[4] = h.n_silhouettes;
+    info[5] = h.n_raw_silhouettes;
+    info[6] = h.max_depth;
+    info[7] = 0;
+    return GQ_OK;
+}
+
+static int copy_out(const std::vector<float> &v, float *out, int64_t *n_floats) {
+    if (!n_floats) return fail(GQ_ERR_INVALID, "n_floats is null");
+    if (out) {
+        if (*n_floats < (int64_t)v.size()) return fail(GQ_ERR_INVALID, "output buffer too small");
+        if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(float));
+    }
+    *n_floats = (int64_t)v.size();
+    return GQ_OK;
+}
+
+int gq_structure_node_data(const gq_structure *s, float *out, int64_t *n) {
+    if (!s) return fail(GQ_ERR_INVALID, "null structure");
+    return copy_out(s->h.bvh_node_data, out, n);
+}
+int gq_structure_snch_node_data(const gq_structure *s, float *out, int64_t *n) {
+    if (!s) return fail(GQ_ERR_INVALID, "null structure");
+    if (!s->h.is_snch) return fail(GQ_ERR_UNSUPPORTED, "structure is not an SNCH");
+    return copy_out(s->h.snch_node_data, out, n);
+}
+int gq_structure_primitive_data(const gq_structure *s, float *out, int64_t *n) {
+    if (!s) return fail(GQ_ERR_INVALID, "null structure");
+    return copy_out(s->h.primitive_data, out, n);
+}
+int gq_structure_silhouette_data(const gq_structure *s, float *out, int64_t *n) {
+    if (!s) return fail(GQ_ERR_INVALID, "null structure");
+    if (!s->h.is_snch) return fail(GQ_ERR_UNSUPPORTED, "structure is not an SNCH");
+    return copy_out(s->h.silhouette_data, out, n);
+}
+int gq_structure_raw_silhouette_data(const gq_structure *s, float *out, int64_t *n) {
+    if (!s) return fail(GQ_ERR_INVALID, "null structure");
+    if (!s->h.is_snch) return fail(GQ_ERR_UNSUPPORTED, "structure is not an SNCH");
+    return copy_out(s->h.raw_silhouette_data, out, n);
+}
+
+int gq_build_flat_silhouette_edges(const float *vertices, int64_t n_vertices, const int32_t *faces,
+                                   int64_t n_faces, float *out, int64_t *n_floats) {
+    std::vector<float> v;
+    std::string err;
+    try {
+        if (!build_flat_silhouette_edges(vertices, n_vertices, faces, n_faces, v, err))
+            return fail(GQ_ERR_INVALID, err);
+    } catch (const std::bad_alloc &) {
+        return fail(GQ_ERR_OOM, "host allocation failed");
+    }
+    return copy_out(v, out, n_floats);
+}
+
+int gq_scene_destroy(gq_scene *scene) {
+    if (!scene) return GQ_OK;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipSetDevice(scene->device);
+    for (void *p : scene->allocs) hipFree(p);
+    scene->ws.release();
+    hipSetDevice(dev);
+    delete scene;
+    return GQ_OK;
+}
+
+int gq_scene_create(const gq_structure *st, gq_scene **out) {
+    if (!st || !out) return fail(GQ_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const HostStructure &h = st->h;
+    std::unique_ptr<gq_scene> s(new (std::nothrow) gq_scene());
+    if (!s) return fail(GQ_ERR_OOM, "host allocation failed");
+    hipError_t e = hipGetDevice(&s->device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    s->dim = h.dim;
+    s->has_snch = h.is_snch;
+    s->max_depth = h.max_depth;
+    PlainView &pv = s->plain;
+    pv.dim = h.dim;
+    pv.n_nodes = h.n_nodes;
+    pv.n_prims = h.n_prims;
+    pv.n_sils = h.n_silhouettes;
+    int rc;
+    auto cleanup = [&](int code) {
+        gq_scene_destroy(s.release());
+        return code;
+    };
+    if ((rc = upload(s.get(), h.bvh_node_data.data(), h.bvh_node_data.size(), &pv.bvh_nodes))) return cleanup(rc);
+    if ((rc = upload(s.get(), h.primitive_data.data(), h.primitive_data.size(), &pv.prims))) return cleanup(rc);
+    if (h.is_snch) {
+        if ((rc = upload(s.get(), h.snch_node_data.data(), h.snch_node_data.size(), &pv.snch_nodes))) return cleanup(rc);
+        if ((rc = upload(s.get(), h.silhouette_data.data(), h.silhouette_data.size(), &pv.sils))) return cleanup(rc);
+    }
+    // Engine layouts.
+    WideHost wh;
+    try {
+        build_wide(h, wh);
+    } catch (const std::bad_alloc &) {
+        return cleanup(fail(GQ_ERR_OOM, "host allocation failed while building wide nodes"));
+    }
+    WideView &wv = s->wide;
+    wv.dim = h.dim;
+    wv.width = wh.width;
+    wv.n_nodes = (int64_t)wh.n_nodes;
+    wv.n_snch_nodes = (int64_t)wh.n_snch_nodes;
+    for (int k = 0; k < 3; ++k) { wv.scene_lo[k] = wh.scene_lo[k]; wv.scene_hi[k] = wh.scene_hi[k]; }
+    if ((rc = upload(s.get(), wh.nodes.data(), wh.nodes.size(), (const uint8_t **)&wv.nodes))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.qnodes.data(), wh.qnodes.size(), (const uint8_t **)&wv.qnodes))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.prims.data(), wh.prims.size(), &wv.prims))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.prim_ids.data(), wh.prim_ids.size(), &wv.prim_ids))) return cleanup(rc);
+    if (h.is_snch) {
+        if ((rc = upload(s.get(), wh.snodes.data(), wh.snodes.size(), (const uint8_t **)&wv.snodes))) return cleanup(rc);
+        if ((rc = upload(s.get(), wh.qsnodes.data(), wh.qsnodes.size(), (const uint8_t **)&wv.qsnodes))) return cleanup(rc);
+        if ((rc = upload(s.get(), wh.sils.data(), wh.sils.size(), &wv.sils))) return cleanup(rc);
+        if ((rc = upload(s.get(), wh.sil_ids.data(), wh.sil_ids.size(), &wv.sil_ids))) return cleanup(rc);
+    }
+    *out = s.release();
+    return GQ_OK;
+}
+
+int gq_scene_info(const gq_scene *s, int64_t info[8]) {
+    if (!s || !info) return fail(GQ_ERR_INVALID, "null argument");
+    info[0] = s->dim;
+    info[1] = s->has_snch ? 1 : 0;
+    info[2] = (int64_t)s->bytes;
+    info[3] = s->plain.n_nodes;
+    info[4] = s->wide.n_nodes;
+    info[5] = s->plain.n_prims;
+    info[6] = s->plain.n_sils;
+    info[7] = 0;
+    return GQ_OK;
+}
+
+static int run_query(gq_scene *s, QueryKind kind, int method, const QueryArgs &q, void *stream) {
+    if (!s) return fail(GQ_ERR_INVALID, "null scene");
+    if (q.n < 0) return fail(GQ_ERR_INVALID, "negative query count");
+    if (q.n == 0) return GQ_OK;
+    if (q.n > (int64_t)INT32_MAX) return fail(GQ_ERR_INVALID, "more than 2^31-1 queries in one call");
+    if (!q.x || (kind == kIntersect && !q.v)) return fail(GQ_ERR_INVALID, "null query pointer");
+    if (kind == kSilhouette && !s->has_snch)
+        return fail(GQ_ERR_UNSUPPORTED, "closest_silhouette_snch needs a scene built from an SNCH");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    if (method == GQ_METHOD_PLAIN) {
+        if (s->max_depth + 2 > 64)
+            return fail(GQ_ERR_UNSUPPORTED, "tree deeper than the plain kernels' 64-entry stack");
+        e = launch_plain(kind, s->plain, q, st);
+    } else if (method == GQ_METHOD_ENGINE || method == GQ_METHOD_ENGINE_FP32) {
+        e = launch_engine(kind, s->wide, method == GQ_METHOD_ENGINE, q, s->ws, st);
+    } else {
+        return fail(GQ_ERR_INVALID, "unknown method");
+    }
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return GQ_OK;
+}
+
+int gq_closest_point_bvh(gq_scene *scene, int method, const float *x, int64_t n,
+                         const gq_records *out, void *stream) {
+    if (!out) return fail(GQ_ERR_INVALID, "null records");
+    QueryArgs q{x, nullptr, nullptr, __builtin_inff(), n, *out};
+    return run_query(scene, kClosestPoint, method, q, stream);
+}
+
+int gq_intersect_bvh(gq_scene *scene, int method, const float *x, const float *v,
+                     const float *r_max, float r_max_scalar, int64_t n, const gq_records *out,
+                     void *stream) {
+    if (!out) return fail(GQ_ERR_INVALID, "null records");
+    QueryArgs q{x, v, r_max, r_max_scalar, n, *out};
+    return run_query(scene, kIntersect, method, q, stream);
+}
+
+int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, const float *r_max,
+                               float r_max_scalar, int64_t n, const gq_records *out, void *stream) {
+    if (!out) return fail(GQ_ERR_INVALID, "null records");
+    QueryArgs q{x, nullptr, r_max, r_max_scalar, n, *out};
+    return run_query(scene, kSilhouette, method, q, stream);
+}
+
+int gq_profile_begin(void) {
+    Prof &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    for (auto &e : p.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    p.ev.clear();
+    p.name.clear();
+    p.on = true;
+    return GQ_OK;
+}
+
+int gq_profile_end(double *total_ms, int64_t *launches, char *name, int32_t name_len) {
+    Prof &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    p.on = false;
+    double tot = 0.0;
+    for (auto &e : p.ev) {
+        hipError_t r = hipEventSynchronize(e.second);
+        if (r != hipSuccess) return hip_fail(r, "hipEventSynchronize");
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e.first, e.second);
+        tot += ms;
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (int64_t)p.ev.size();
+    if (name && name_len > 0) {
+        std::strncpy(name, p.name.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    p.ev.clear();
+    return GQ_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,58 @@
+// Internal definitions shared by the C-ABI layer and the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/gquery_amd.h"
+
+namespace gqa {
+
+// Device view of a scene in the reference layout (flat float32 AoS records).
+struct PlainView {
+    int dim;
+    int64_t n_nodes, n_prims, n_sils;
+    const float *bvh_nodes;   // stride 7 / 9
+    const float *prims;       // stride 5 / 10
+    const float *snch_nodes;  // stride 13 / 16 (nullptr for a BVH-only scene)
+    const float *sils;        // stride 10 / 19
+};
+
+// Engine layout (built on the host by wide.cpp, uploaded by the API layer).
+struct WideView {
+    int dim;
+    int width;                // children per node (4 or 8)
+    int64_t n_nodes, n_snch_nodes;
+    const void *nodes;        // fp32 wide nodes
+    const void *qnodes;       // quantized wide nodes
+    const void *snodes;       // fp32 wide SNCH nodes (boxes + cones)
+    const void *qsnodes;      // quantized wide SNCH nodes
+    const float4 *prims;      // 3D: 3 float4 per triangle (a|id, b, c); 2D: 1 float4 per segment
+    const int32_t *prim_ids;  // original id per ordered slot
+    const float4 *sils;       // engine silhouette records
+    const int32_t *sil_ids;
+    float scene_lo[3], scene_hi[3];
+};
+
+struct QueryArgs {
+    const float *x;
+    const float *v;           // rays only
+    const float *r_max;       // optional per-query
+    float r_max_scalar;
+    int64_t n;
+    gq_records out;
+};
+
+enum QueryKind { kClosestPoint = 0, kIntersect = 1, kSilhouette = 2 };
+
+hipError_t launch_plain(QueryKind kind, const PlainView &s, const QueryArgs &q, hipStream_t st);
+
+void set_error(const std::string &msg);
+
+// Traversal-kernel timing (gq_profile_begin / gq_profile_end); no-ops unless enabled.
+void prof_begin(hipStream_t st);
+void prof_end(hipStream_t st, const char *kernel);
+
+} // namespace gqa

@@ -1,0 +1,122 @@
+"""Host-side builders with the surface of gQuery's nanobind module gquery_ext
+(src/gquery.cpp:180-250): BVH / BVH3D / SNCH / SNCH3D constructors taking vertices and indices,
+and node_data() / primitive_data() / silhouette_data() / raw_silhouette_data() returning the flat
+float32 arrays of the reference wire format; build_flat_silhouette_edges(vertices, faces).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def _as_rows(a, dim: int, dtype) -> np.ndarray:
+    """(N, dim) C-contiguous array from numpy/torch/sequence input; (dim, N) is transposed."""
+    try:
+        import torch
+        if isinstance(a, torch.Tensor):
+            a = a.detach().cpu().numpy()
+    except ImportError:  # pragma: no cover
+        pass
+    a = np.asarray(a)
+    if a.ndim != 2:
+        raise ValueError(f"expected a 2-D array of {dim}-vectors, got shape {a.shape}")
+    if a.shape[1] != dim and a.shape[0] == dim:
+        a = a.T
+    if a.shape[1] != dim:
+        raise ValueError(f"expected shape (N, {dim}), got {a.shape}")
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class _Structure:
+    _dim = 0
+    _snch = False
+
+    def __init__(self, vertices, indices, max_prims_in_node: int = 10):
+        v = _as_rows(vertices, self._dim, np.float32)
+        f = _as_rows(indices, self._dim, np.int32)
+        self._vertices, self._indices = v, f
+        h = C.c_void_p()
+        fn = _lib.lib().gq_build_snch if self._snch else _lib.lib().gq_build_bvh
+        _lib.check(fn(self._dim, v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0],
+                      int(max_prims_in_node), C.byref(h)), type(self).__name__)
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.lib().gq_structure_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 8)()
+        _lib.check(_lib.lib().gq_structure_info(self._h, a))
+        keys = ["dim", "is_snch", "n_nodes", "n_prims", "n_silhouettes", "n_raw_silhouettes", "max_depth"]
+        return dict(zip(keys, list(a)[:7]))
+
+    def _fetch(self, fn_name: str) -> np.ndarray:
+        fn = getattr(_lib.lib(), fn_name)
+        n = C.c_int64(0)
+        _lib.check(fn(self._h, None, C.byref(n)), fn_name)
+        out = np.empty(n.value, dtype=np.float32)
+        if n.value:
+            _lib.check(fn(self._h, out.ctypes.data, C.byref(n)), fn_name)
+        return out
+
+    def node_data(self) -> np.ndarray:
+        return self._fetch("gq_structure_snch_node_data" if self._snch else "gq_structure_node_data")
+
+    def bvh_node_data(self) -> np.ndarray:
+        return self._fetch("gq_structure_node_data")
+
+    def primitive_data(self) -> np.ndarray:
+        return self._fetch("gq_structure_primitive_data")
+
+
+class _SnchMixin:
+    def silhouette_data(self) -> np.ndarray:
+        return self._fetch("gq_structure_silhouette_data")
+
+    def raw_silhouette_data(self) -> np.ndarray:
+        return self._fetch("gq_structure_raw_silhouette_data")
+
+
+class BVH(_Structure):
+    """gquery_ext.BVH (2D line segments)."""
+    _dim = 2
+
+
+class BVH3D(_Structure):
+    """gquery_ext.BVH3D (triangles)."""
+    _dim = 3
+
+
+class SNCH(_SnchMixin, _Structure):
+    """gquery_ext.SNCH (2D): BVH + normal cones + silhouette vertices."""
+    _dim = 2
+    _snch = True
+
+
+class SNCH3D(_SnchMixin, _Structure):
+    """gquery_ext.SNCH3D: BVH + normal cones + silhouette edges."""
+    _dim = 3
+    _snch = True
+
+
+def build_flat_silhouette_edges(vertices, faces) -> np.ndarray:
+    """gquery_ext.build_flat_silhouette_edges: all mesh edges as 19-float silhouette records."""
+    v = _as_rows(vertices, 3, np.float32)
+    f = _as_rows(faces, 3, np.int32)
+    n = C.c_int64(0)
+    L = _lib.lib()
+    _lib.check(L.gq_build_flat_silhouette_edges(v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0], None, C.byref(n)))
+    out = np.empty(n.value, dtype=np.float32)
+    _lib.check(L.gq_build_flat_silhouette_edges(v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0],
+                                                out.ctypes.data, C.byref(n)))
+    return out

@@ -1,0 +1,151 @@
+"""Device-resident scene (gq_scene) and the three query calls on torch tensors.
+
+Inputs may be torch tensors on any device, numpy arrays or sequences; they are moved to the
+scene's HIP device as contiguous float32 (N, dim).  Outputs are freshly allocated device tensors.
+All calls are ordered on torch's current stream of the scene's device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .shapes import primitive as rec
+
+INF = float("inf")
+
+
+def _device(device=None) -> torch.device:
+    if device is None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("gquery_amd needs a HIP device (torch.cuda.is_available() is False)")
+        return torch.device("cuda", torch.cuda.current_device())
+    return torch.device(device)
+
+
+def as_points(x, dim: int, device: torch.device) -> torch.Tensor:
+    if not isinstance(x, torch.Tensor):
+        x = torch.as_tensor(np.asarray(x, dtype=np.float32))
+    if x.ndim == 1 and x.shape[0] == dim:
+        x = x[None, :]
+    if x.ndim != 2:
+        raise ValueError(f"expected (N, {dim}) points, got shape {tuple(x.shape)}")
+    if x.shape[1] != dim and x.shape[0] == dim:
+        x = x.T
+    if x.shape[1] != dim:
+        raise ValueError(f"expected (N, {dim}) points, got shape {tuple(x.shape)}")
+    return x.to(device=device, dtype=torch.float32).contiguous()
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Scene:
+    """One polyline or mesh on the device: reference-layout BVH/SNCH and the engine layouts."""
+
+    def __init__(self, structure, device=None):
+        self.device = _device(device)
+        self.dim = structure.info()["dim"]
+        self.has_snch = bool(structure.info()["is_snch"])
+        self._structure = structure
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().gq_scene_create(structure.handle, C.byref(h)), "gq_scene_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                torch.cuda.synchronize(self.device)
+            except Exception:
+                pass
+            _lib.lib().gq_scene_destroy(h)
+            self._h = None
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 8)()
+        _lib.check(_lib.lib().gq_scene_info(self._h, a))
+        keys = ["dim", "has_snch", "device_bytes", "n_nodes", "n_wide_nodes", "n_prims", "n_silhouettes"]
+        return dict(zip(keys, list(a)[:7]))
+
+    # -- helpers -------------------------------------------------------------------------------
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _alloc(self, n: int, fields):
+        D = self.dim
+        f32 = dict(device=self.device, dtype=torch.float32)
+        out = {}
+        for name in fields:
+            if name in ("valid", "on_boundary"):
+                out[name] = torch.empty(n, device=self.device, dtype=torch.bool)
+            elif name in ("p", "n"):
+                out[name] = torch.empty((n, D), **f32)
+            elif name == "uv":
+                out[name] = torch.empty((n, 2), **f32) if D == 3 else torch.empty(n, **f32)
+            elif name == "d":
+                out[name] = torch.empty(n, **f32)
+            elif name == "prim_id":
+                out[name] = torch.empty(n, device=self.device, dtype=torch.int32)
+        r = _lib.Records()
+        for name, t in out.items():
+            setattr(r, name, t.data_ptr())
+        return out, r
+
+    def _r_max(self, r_max, n):
+        if isinstance(r_max, (int, float)):
+            return None, float(r_max)
+        t = torch.as_tensor(r_max, dtype=torch.float32)
+        if t.numel() == 1:
+            return None, float(t.reshape(()).item())
+        t = t.to(self.device).reshape(-1).contiguous()
+        if t.numel() != n:
+            raise ValueError("r_max must be a scalar or have one entry per query")
+        return t, INF
+
+    # -- queries -------------------------------------------------------------------------------
+    def closest_point(self, x, method: str = "engine"):
+        m = _lib.METHODS[method]
+        with torch.cuda.device(self.device):
+            x = as_points(x, self.dim, self.device)
+            n = x.shape[0]
+            out, r = self._alloc(n, ["valid", "p", "n", "uv", "d", "prim_id"])
+            _lib.check(_lib.lib().gq_closest_point_bvh(self._h, m, _ptr(x), n, C.byref(r), self._stream()),
+                       "closest_point_bvh")
+        if self.dim == 3:
+            return rec.ClosestPointRecord3D(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"])
+        return rec.ClosestPointRecord(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"])
+
+    def intersect(self, x, v, r_max=INF, method: str = "engine"):
+        m = _lib.METHODS[method]
+        with torch.cuda.device(self.device):
+            x = as_points(x, self.dim, self.device)
+            v = as_points(v, self.dim, self.device)
+            if x.shape != v.shape:
+                raise ValueError("origins and directions must have the same shape")
+            n = x.shape[0]
+            rt, rs = self._r_max(r_max, n)
+            out, r = self._alloc(n, ["valid", "p", "n", "uv", "d", "prim_id", "on_boundary"])
+            _lib.check(_lib.lib().gq_intersect_bvh(self._h, m, _ptr(x), _ptr(v), _ptr(rt), rs, n, C.byref(r),
+                                                   self._stream()), "intersect_bvh")
+        if self.dim == 3:
+            return rec.Intersection3D(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"],
+                                      out["on_boundary"])
+        return rec.Intersection(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"],
+                                out["on_boundary"])
+
+    def closest_silhouette(self, x, r_max=INF, method: str = "engine"):
+        m = _lib.METHODS[method]
+        with torch.cuda.device(self.device):
+            x = as_points(x, self.dim, self.device)
+            n = x.shape[0]
+            rt, rs = self._r_max(r_max, n)
+            out, r = self._alloc(n, ["valid", "p", "d", "prim_id"])
+            _lib.check(_lib.lib().gq_closest_silhouette_snch(self._h, m, _ptr(x), _ptr(rt), rs, n, C.byref(r),
+                                                             self._stream()), "closest_silhouette_snch")
+        cls = rec.ClosestSilhouettePointRecord3D if self.dim == 3 else rec.ClosestSilhouettePointRecord
+        return cls(out["valid"], out["p"], out["d"], out["prim_id"])

@@ -1,0 +1,128 @@
+/* gquery_amd -- C ABI of the MI355X traversal engine for gQuery's BVH / SNCH queries.
+ *
+ * The library replaces two layers of gQuery:
+ *   1. the nanobind extension gquery_ext (src/gquery.cpp:180-250) that builds the BVH / SNCH on
+ *      the CPU and hands flat float32 arrays (node_data / primitive_data / silhouette_data) to
+ *      Python -> gq_build_* / gq_structure_* / gq_build_flat_silhouette_edges;
+ *   2. the DrJit-traced traversals BVH.closest_point / BVH.intersect / SNCH.closest_silhouette and
+ *      their 3D twins (gquery/shapes/{bvh,bvh3d,snch,snch3d}.py) -> gq_closest_point_bvh /
+ *      gq_intersect_bvh / gq_closest_silhouette_snch, run as HIP kernels on a device scene.
+ *
+ * Conventions: plain pointers and sizes only.  Query inputs and record outputs are device
+ * pointers (any hipMalloc'd / torch memory of the current device) laid out row-major:
+ * a point batch is n x dim float32, records are structure-of-arrays (one pointer per field).
+ * `stream` is a hipStream_t passed as void* (NULL = default stream); every query call is
+ * stream-ordered and returns as soon as its work is enqueued.  Every function returns GQ_OK (0) or
+ * a negative status; gq_last_error() gives the message of the last failure on the calling thread.
+ * Scene buffers are read-only after gq_scene_create, so concurrent queries on distinct streams
+ * against one scene are safe.
+ */
+#ifndef GQUERY_AMD_H
+#define GQUERY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GQ_OK 0
+#define GQ_ERR_INVALID (-1)     /* bad argument (null pointer, wrong dim, index out of range) */
+#define GQ_ERR_OOM (-2)         /* device or host allocation failed */
+#define GQ_ERR_DEVICE (-3)      /* HIP runtime error */
+#define GQ_ERR_UNSUPPORTED (-4) /* query not available on this scene (e.g. silhouettes of a BVH) */
+
+/* Traversal method of a query call. */
+#define GQ_METHOD_PLAIN 0       /* reference layout: binary AoS float32 nodes, lane per query */
+#define GQ_METHOD_ENGINE_FP32 1 /* wide SoA nodes with float32 bounds, sorted queries */
+#define GQ_METHOD_ENGINE 2      /* wide nodes with 8-bit (3D) / 16-bit (2D) quantized bounds+cones */
+
+typedef struct gq_structure gq_structure; /* host-side BVH or SNCH (CPU builder output) */
+typedef struct gq_scene gq_scene;         /* device-resident scene */
+
+/* Output records, one field per pointer (a field may be NULL to skip it).  Replaces the DrJit
+ * dataclasses of gquery/shapes/primitive.py:9-84.  Misses are all-zero (valid = 0, d = 0,
+ * prim_id = 0), as dr.zeros(record) is in the reference (bvh.py:194, bvh3d.py:170). */
+typedef struct gq_records {
+    uint8_t *valid;       /* n                                                         */
+    float *p;             /* n x dim                                                   */
+    float *n;             /* n x dim   (closest point / intersection only)             */
+    float *uv;            /* 3D: n x 2 uv; 2D: n  t   (closest point / intersection)   */
+    float *d;             /* n                                                         */
+    int32_t *prim_id;     /* n         original primitive / silhouette id               */
+    uint8_t *on_boundary; /* n         (intersection only)                             */
+} gq_records;
+
+int gq_version(void);
+const char *gq_last_error(void);
+
+/* ---- CPU builders (replace gquery_ext.BVH / BVH3D / SNCH / SNCH3D, src/gquery.cpp:181-247) ---- */
+
+/* vertices: n_vertices x dim float32; indices: n_prims x dim int32 (segments in 2D, triangles in
+ * 3D).  max_prims_in_node defaults to 10 in the reference (include/shapes/bvh.h:158-168). */
+int gq_build_bvh(int dim, const float *vertices, int64_t n_vertices, const int32_t *indices,
+                 int64_t n_prims, int32_t max_prims_in_node, gq_structure **out);
+int gq_build_snch(int dim, const float *vertices, int64_t n_vertices, const int32_t *indices,
+                  int64_t n_prims, int32_t max_prims_in_node, gq_structure **out);
+void gq_structure_free(gq_structure *s);
+
+/* info[0..7] = dim, is_snch, n_nodes, n_prims, n_silhouettes, n_raw_silhouettes, max_depth, 0 */
+int gq_structure_info(const gq_structure *s, int64_t info[8]);
+
+/* Copies of the flat float32 arrays in the reference wire format (bvh.cpp:329-353,
+ * snch.cpp:248-285).  `out` must hold the number of floats reported by passing out = NULL
+ * (the count is written to *n_floats). */
+int gq_structure_node_data(const gq_structure *s, float *out, int64_t *n_floats);      /* BVH.node_data  */
+int gq_structure_snch_node_data(const gq_structure *s, float *out, int64_t *n_floats); /* SNCH.node_data */
+int gq_structure_primitive_data(const gq_structure *s, float *out, int64_t *n_floats);
+int gq_structure_silhouette_data(const gq_structure *s, float *out, int64_t *n_floats);
+int gq_structure_raw_silhouette_data(const gq_structure *s, float *out, int64_t *n_floats);
+
+/* gquery_ext.build_flat_silhouette_edges (src/shapes/silhouette_edge.cpp:62-71). */
+int gq_build_flat_silhouette_edges(const float *vertices, int64_t n_vertices, const int32_t *faces,
+                                   int64_t n_faces, float *out, int64_t *n_floats);
+
+/* ---- Device scenes (replace the Float(...) uploads of bvh.py:124-125, snch.py:100-104) ---- */
+
+/* Uploads every layout of `s` to the current HIP device.  A BVH structure supports the closest
+ * point and ray queries; an SNCH structure supports all three. */
+int gq_scene_create(const gq_structure *s, gq_scene **out);
+int gq_scene_destroy(gq_scene *scene);
+/* info[0..7] = dim, has_snch, device bytes, n_nodes, n_wide_nodes, n_prims, n_silhouettes, 0 */
+int gq_scene_info(const gq_scene *scene, int64_t info[8]);
+
+/* ---- Queries ---- */
+
+/* Polyline/Mesh.closest_point_bvh (polyline.py:171-173, mesh.py:119-121) ->
+ * BVH.closest_point (bvh.py:127-198) / BVH3D.closest_point (bvh3d.py:108-174).
+ * Fills valid, p, n, uv (3D) / t (2D), d, prim_id. */
+int gq_closest_point_bvh(gq_scene *scene, int method, const float *x, int64_t n,
+                         const gq_records *out, void *stream);
+
+/* Polyline/Mesh.intersect_bvh (polyline.py:132-135, mesh.py:86-89) -> BVH.intersect
+ * (bvh.py:200-276) / BVH3D.intersect (bvh3d.py:176-239).  r_max: per-query device array, or NULL
+ * to use r_max_scalar for every ray.  Fills valid, p, n, uv/t, d, prim_id, on_boundary. */
+int gq_intersect_bvh(gq_scene *scene, int method, const float *x, const float *v,
+                     const float *r_max, float r_max_scalar, int64_t n, const gq_records *out,
+                     void *stream);
+
+/* Polyline/Mesh.closest_silhouette_snch (polyline.py:225-227, mesh.py:137-139) ->
+ * SNCH.closest_silhouette (snch.py:106-164) / SNCH3D.closest_silhouette (snch3d.py:143-203).
+ * Fills valid, p, d, prim_id. */
+int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, const float *r_max,
+                               float r_max_scalar, int64_t n, const gq_records *out, void *stream);
+
+/* ---- Kernel timing (replaces the time.time() brackets around dr.eval in the reference demos,
+ * demo/closest_point_demo_3d.py:103-107) ---- */
+
+/* Starts recording a pair of hipEvents around every traversal kernel launched from now on (on the
+ * stream of its query call).  Clears earlier records. */
+int gq_profile_begin(void);
+/* Stops recording, waits for the recorded events and returns the summed traversal-kernel time,
+ * the number of launches and the kernel's name (truncated to name_len bytes). */
+int gq_profile_end(double *total_ms, int64_t *launches, char *name, int32_t name_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GQUERY_AMD_H */

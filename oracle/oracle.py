@@ -1,0 +1,141 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes wrapper of oracle/liboracle.so (oracle/gq_oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this module, as the
+checker; the product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    return LIB
+
+
+class _Out(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in
+                ["valid", "p", "n", "uv", "d", "prim_id", "on_boundary", "visits", "child_tests", "leaf_tests"]]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        h = C.CDLL(str(LIB))
+        vp, i64, f32 = C.c_void_p, C.c_int64, C.c_float
+        h.or_closest_point.argtypes = [C.c_int, vp, vp, vp, i64, _Out]
+        h.or_intersect.argtypes = [C.c_int, vp, vp, vp, vp, vp, f32, i64, _Out]
+        h.or_silhouette.argtypes = [C.c_int, vp, vp, vp, vp, f32, i64, _Out]
+        for f in (h.or_closest_point, h.or_intersect, h.or_silhouette):
+            f.restype = i64
+        h.or_bf_closest_point_f64.argtypes = [C.c_int, vp, vp, i64, vp, i64, vp, vp, vp, vp]
+        h.or_bf_intersect_f64.argtypes = [C.c_int, vp, vp, i64, vp, vp, i64, vp, vp, vp]
+        h.or_bf_silhouette_f64.argtypes = [C.c_int, vp, i64, vp, i64, vp, vp, vp, vp]
+        h.or_num_threads.restype = C.c_int
+        _lib = h
+    return _lib
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _records(n, dim, kind):
+    r = {
+        "valid": np.zeros(n, np.uint8), "p": np.zeros((n, dim), np.float32), "d": np.zeros(n, np.float32),
+        "prim_id": np.zeros(n, np.int32),
+        "visits": np.zeros(n, np.int32), "child_tests": np.zeros(n, np.int32), "leaf_tests": np.zeros(n, np.int32),
+    }
+    if kind != "sil":
+        r["n"] = np.zeros((n, dim), np.float32)
+        r["uv"] = np.zeros((n, 2), np.float32) if dim == 3 else np.zeros(n, np.float32)
+    if kind == "ray":
+        r["on_boundary"] = np.zeros(n, np.uint8)
+    o = _Out()
+    for k, v in r.items():
+        setattr(o, k, v.ctypes.data)
+    return r, o
+
+
+def closest_point(dim, node_data, primitive_data, x):
+    x = _c(x, np.float32)
+    nodes, prims = _c(node_data, np.float32), _c(primitive_data, np.float32)
+    r, o = _records(x.shape[0], dim, "cp")
+    ov = lib().or_closest_point(dim, nodes.ctypes.data, prims.ctypes.data, x.ctypes.data, x.shape[0], o)
+    r["overflows"] = ov
+    return r
+
+
+def intersect(dim, node_data, primitive_data, x, v, r_max=float("inf")):
+    x, v = _c(x, np.float32), _c(v, np.float32)
+    nodes, prims = _c(node_data, np.float32), _c(primitive_data, np.float32)
+    r, o = _records(x.shape[0], dim, "ray")
+    rm = None
+    if np.ndim(r_max) > 0:
+        rm = _c(r_max, np.float32)
+    ov = lib().or_intersect(dim, nodes.ctypes.data, prims.ctypes.data, x.ctypes.data, v.ctypes.data,
+                            None if rm is None else rm.ctypes.data, float(r_max) if rm is None else 0.0,
+                            x.shape[0], o)
+    r["overflows"] = ov
+    return r
+
+
+def silhouette(dim, snch_node_data, silhouette_data, x, r_max=float("inf")):
+    x = _c(x, np.float32)
+    nodes, sils = _c(snch_node_data, np.float32), _c(silhouette_data, np.float32)
+    r, o = _records(x.shape[0], dim, "sil")
+    rm = None
+    if np.ndim(r_max) > 0:
+        rm = _c(r_max, np.float32)
+    ov = lib().or_silhouette(dim, nodes.ctypes.data, sils.ctypes.data, x.ctypes.data,
+                             None if rm is None else rm.ctypes.data, float(r_max) if rm is None else 0.0,
+                             x.shape[0], o)
+    r["overflows"] = ov
+    return r
+
+
+def bf_closest_point(dim, V, F, x):
+    V, F, x = _c(V, np.float32), _c(F, np.int32), _c(x, np.float32)
+    n = x.shape[0]
+    d, p, i, d2 = np.zeros(n), np.zeros((n, dim)), np.zeros(n, np.int32), np.zeros(n)
+    lib().or_bf_closest_point_f64(dim, V.ctypes.data, F.ctypes.data, F.shape[0], x.ctypes.data, n,
+                                  d.ctypes.data, p.ctypes.data, i.ctypes.data, d2.ctypes.data)
+    return {"d": d, "p": p, "prim_id": i, "second_d": d2}
+
+
+def bf_intersect(dim, V, F, x, v):
+    V, F, x, v = _c(V, np.float32), _c(F, np.int32), _c(x, np.float32), _c(v, np.float32)
+    n = x.shape[0]
+    t, i, t2 = np.zeros(n), np.zeros(n, np.int32), np.zeros(n)
+    lib().or_bf_intersect_f64(dim, V.ctypes.data, F.ctypes.data, F.shape[0], x.ctypes.data, v.ctypes.data, n,
+                              t.ctypes.data, i.ctypes.data, t2.ctypes.data)
+    return {"t": t, "prim_id": i, "second_t": t2}
+
+
+def bf_silhouette(dim, raw_silhouette_data, x):
+    raw, x = _c(raw_silhouette_data, np.float32), _c(x, np.float32)
+    ss = 10 if dim == 2 else 19
+    n = x.shape[0]
+    d, p, i, d2 = np.zeros(n), np.zeros((n, dim)), np.zeros(n, np.int32), np.zeros(n)
+    lib().or_bf_silhouette_f64(dim, raw.ctypes.data, raw.size // ss, x.ctypes.data, n,
+                               d.ctypes.data, p.ctypes.data, i.ctypes.data, d2.ctypes.data)
+    return {"d": d, "p": p, "prim_id": i, "second_d": d2}
+
+
+def num_threads() -> int:
+    return lib().or_num_threads()

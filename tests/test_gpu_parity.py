@@ -1,0 +1,151 @@
+"""GPU parity: every query path through the C ABI on the device, against the CPU oracle.
+
+Plain kernels restate the reference traversal in the same arithmetic as the oracle, so they must
+agree with it bit for bit (transcendentals in the SNCH cone test aside).  Engine kernels must meet
+the acceptance rule of tests/parity.py against the oracle and the FP64 brute force."""
+import numpy as np
+import pytest
+
+from gquery_amd import ext, synth
+from oracle import oracle as O
+from tests import parity
+from tests.test_builder import PENTAGON_E, PENTAGON_V, TET_F, TET_V
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["plain", "engine_fp32", "engine"]
+
+_SCENES = {}
+
+
+def scene(name):
+    """Cached (V, F, Mesh/Polyline) for a named synthetic scene."""
+    if name not in _SCENES:
+        from gquery_amd.shapes.mesh import Mesh
+        from gquery_amd.shapes.polyline import Polyline
+        if name.startswith("star"):
+            V, F = synth.star_polyline(int(name[4:]))
+            _SCENES[name] = (V, F, Polyline(V, F, device="cuda:0"))
+        else:
+            V, F = synth.displaced_icosphere(int(name[3:]))
+            _SCENES[name] = (V, F, Mesh(V, F, device="cuda:0"))
+    return _SCENES[name]
+
+
+def _np(r, fields=("valid", "prim_id", "d", "p", "n", "uv", "on_boundary")):
+    import torch
+    torch.cuda.synchronize()
+    return {f: getattr(r, f).cpu().numpy() for f in fields if hasattr(r, f)}
+
+
+def _oracle(kind, dim, s, x, v=None, r_max=float("inf")):
+    if kind == "cp":
+        return O.closest_point(dim, s.bvh_node_data(), s.primitive_data(), x)
+    if kind == "ray":
+        return O.intersect(dim, s.bvh_node_data(), s.primitive_data(), x, v, r_max)
+    return O.silhouette(dim, s.node_data(), s.silhouette_data(), x, r_max)
+
+
+def _query(shape, kind, method, x, v=None, r_max=float("inf")):
+    if kind == "cp":
+        return shape.closest_point_bvh(x, method=method)
+    if kind == "ray":
+        return shape.intersect_bvh(x, v, r_max, method=method)
+    return shape.closest_silhouette_snch(x, r_max, method=method)
+
+
+def _gap_from_bruteforce(kind, dim, s, V, F, x, v):
+    if kind == "cp":
+        b = O.bf_closest_point(dim, V, F, x)
+        return parity.gap_mask(b["d"], b["second_d"])
+    if kind == "ray":
+        b = O.bf_intersect(dim, V, F, x, v)
+        return parity.gap_mask(b["t"], b["second_t"])
+    b = O.bf_silhouette(dim, s.raw_silhouette_data(), x)
+    return parity.gap_mask(b["d"], b["second_d"])
+
+
+@pytest.mark.parametrize("name", ["star1024", "star4096", "ico3", "ico5"])
+@pytest.mark.parametrize("kind", ["cp", "ray", "sil"])
+@pytest.mark.parametrize("method", METHODS)
+def test_parity_vs_oracle(gpu, name, kind, method):
+    V, F, shape = scene(name)
+    dim = V.shape[1]
+    n = 4096
+    x = synth.uniform_points(n, dim, seed=21)
+    v = synth.uniform_directions(n, dim, seed=22)
+    ref = _oracle(kind, dim, shape.snch, x, v)
+    got = _np(_query(shape, kind, method, x, v))
+    gap = _gap_from_bruteforce(kind, dim, shape.snch, V, F, x, v)
+    vec = ("p", "n") if kind != "sil" else ("p",)
+    res = parity.summarize(f"{name}/{kind}/{method}", got, ref, gap, vec_fields=vec)
+    assert res["ok"], res
+    if method == "plain" and kind != "sil":
+        # same arithmetic as the oracle: bit-identical records
+        assert np.array_equal(got["prim_id"], ref["prim_id"])
+        assert np.array_equal(got["d"].view(np.int32), ref["d"].view(np.int32))
+        assert np.array_equal(got["p"].view(np.int32), ref["p"].view(np.int32))
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_known_answers_2d(gpu, method):
+    import torch
+    from gquery_amd.shapes.polyline import Polyline
+    pl = Polyline(PENTAGON_V, PENTAGON_E, device="cuda:0")
+    r = _np(pl.closest_point_bvh(np.array([[0.8, 0.8]], np.float32), method=method))
+    assert r["valid"][0] and r["prim_id"][0] == 2
+    np.testing.assert_allclose(r["p"][0], [1.02, 1.24], atol=1e-6)
+    np.testing.assert_allclose(r["d"][0], np.sqrt(0.242), rtol=1e-6)
+    r = _np(pl.intersect_bvh(np.array([[0.8, 0.8]], np.float32), np.array([[1.0, 1.0]], np.float32), method=method))
+    assert r["valid"][0] and r["prim_id"][0] == 2 and r["on_boundary"][0]
+    np.testing.assert_allclose(r["d"][0], 0.7 - 1 / 3, rtol=1e-6)
+    r = _np(pl.closest_silhouette_snch(torch.tensor([[2.0, 0.5]]), method=method), ("valid", "prim_id", "d", "p"))
+    assert r["valid"][0] and r["prim_id"][0] == 2
+    np.testing.assert_allclose(r["d"][0], np.sqrt(0.5), rtol=1e-6)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_known_answers_3d(gpu, method):
+    from gquery_amd.shapes.mesh import Mesh
+    m = Mesh(TET_V, TET_F, device="cuda:0")
+    r = _np(m.closest_point_bvh(np.array([[0.6, 0.6, 0.6]], np.float32), method=method))
+    assert r["valid"][0] and r["prim_id"][0] == 2
+    np.testing.assert_allclose(r["d"][0], 0.2 / np.sqrt(21), rtol=2e-6)
+    np.testing.assert_allclose(r["p"][0], np.array([59, 61, 62]) / 105, atol=1e-6)
+    r = _np(m.intersect_bvh(np.array([[0.6, 0.6, 0.6]], np.float32), np.array([[1.0, 1.0, 1.0]], np.float32),
+                            method=method))
+    assert not r["valid"][0] and r["d"][0] == 0 and r["prim_id"][0] == 0
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_edge_cases(gpu, method, dim):
+    """Empty batch, single query, queries on vertices / far away, finite and per-query r_max."""
+    import torch
+    name = "star1024" if dim == 2 else "ico3"
+    V, F, shape = scene(name)
+    s = shape.snch
+    # empty batch
+    r = shape.closest_point_bvh(np.zeros((0, dim), np.float32), method=method)
+    assert r.d.shape[0] == 0
+    # vertices, far points, a single query
+    far = np.full((1, dim), 1e4, np.float32)
+    x = np.concatenate([V[:50], far, -far, np.zeros((1, dim), np.float32)])
+    ref = _oracle("cp", dim, s, x)
+    got = _np(shape.closest_point_bvh(x, method=method))
+    assert got["valid"].all()
+    np.testing.assert_allclose(got["d"], ref["d"], rtol=1e-5, atol=1e-6)
+    assert (got["d"][:50] == 0).all()
+    # rays with a finite r_max (scalar and per query) and silhouettes with a finite r_max
+    x = synth.uniform_points(2000, dim, seed=5)
+    v = synth.uniform_directions(2000, dim, seed=6)
+    rm = np.random.default_rng(7).uniform(0.05, 1.0, 2000).astype(np.float32)
+    for r_max in (0.3, rm):
+        ref = _oracle("ray", dim, s, x, v, r_max)
+        got = _np(shape.intersect_bvh(x, v, torch.as_tensor(r_max), method=method))
+        res = parity.summarize("ray r_max", got, ref, None, vec_fields=())
+        assert res["valid_mismatch"] == 0 and res["max_err_d"] < 1e-5, res
+        ref = _oracle("sil", dim, s, x, None, r_max)
+        got = _np(shape.closest_silhouette_snch(x, torch.as_tensor(r_max), method=method), ("valid", "prim_id", "d", "p"))
+        res = parity.summarize("sil r_max", got, ref, None, vec_fields=())
+        assert res["valid_mismatch"] == 0 and res["max_err_d"] < 1e-5, res
