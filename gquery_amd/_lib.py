@@ -57,6 +57,7 @@ SIGNATURES = {
     "gq_closest_point_bvh": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_intersect_bvh": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_closest_silhouette_snch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
+    "gq_structure_engine_layout": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, _i64p]),
     "gq_profile_begin": (C.c_int, []),
     "gq_profile_end": (C.c_int, [C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
 }

@@ -241,18 +241,29 @@ int gq_scene_create(const gq_structure *st, gq_scene **out) {
     WideView &wv = s->wide;
     wv.dim = h.dim;
     wv.width = wh.width;
+    wv.wide_depth = wh.wide_depth;
+    wv.bin_depth = h.max_depth;
     wv.n_nodes = (int64_t)wh.n_nodes;
     wv.n_snch_nodes = (int64_t)wh.n_snch_nodes;
-    for (int k = 0; k < 3; ++k) { wv.scene_lo[k] = wh.scene_lo[k]; wv.scene_hi[k] = wh.scene_hi[k]; }
+    wv.n_prims = h.n_prims;
+    wv.n_sils = h.n_silhouettes;
+    wv.root_ref = wh.root_ref;
+    wv.sroot_ref = wh.sroot_ref;
+    for (int k = 0; k < 3; ++k) {
+        wv.scene_lo[k] = wh.scene_lo[k]; wv.scene_hi[k] = wh.scene_hi[k];
+        wv.sroot_lo[k] = wh.sroot_lo[k]; wv.sroot_hi[k] = wh.sroot_hi[k];
+    }
     if ((rc = upload(s.get(), wh.nodes.data(), wh.nodes.size(), (const uint8_t **)&wv.nodes))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.qnodes.data(), wh.qnodes.size(), (const uint8_t **)&wv.qnodes))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.prims.data(), wh.prims.size(), &wv.prims))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.prim_ids.data(), wh.prim_ids.size(), &wv.prim_ids))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.big_leaves.data(), wh.big_leaves.size(), &wv.big_leaves))) return cleanup(rc);
     if (h.is_snch) {
         if ((rc = upload(s.get(), wh.snodes.data(), wh.snodes.size(), (const uint8_t **)&wv.snodes))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.qsnodes.data(), wh.qsnodes.size(), (const uint8_t **)&wv.qsnodes))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.sils.data(), wh.sils.size(), &wv.sils))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.sil_ids.data(), wh.sil_ids.size(), &wv.sil_ids))) return cleanup(rc);
+        if ((rc = upload(s.get(), wh.ref_cones.data(), wh.ref_cones.size(), &wv.ref_cones))) return cleanup(rc);
     }
     *out = s.release();
     return GQ_OK;
@@ -286,6 +297,8 @@ static int run_query(gq_scene *s, QueryKind kind, int method, const QueryArgs &q
             return fail(GQ_ERR_UNSUPPORTED, "tree deeper than the plain kernels' 64-entry stack");
         e = launch_plain(kind, s->plain, q, st);
     } else if (method == GQ_METHOD_ENGINE || method == GQ_METHOD_ENGINE_FP32) {
+        if (!engine_supported(kind, s->wide))
+            return fail(GQ_ERR_UNSUPPORTED, "tree deeper than the engine kernels' traversal stack");
         e = launch_engine(kind, s->wide, method == GQ_METHOD_ENGINE, q, s->ws, st);
     } else {
         return fail(GQ_ERR_INVALID, "unknown method");
@@ -314,6 +327,39 @@ int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, cons
     if (!out) return fail(GQ_ERR_INVALID, "null records");
     QueryArgs q{x, nullptr, r_max, r_max_scalar, n, *out};
     return run_query(scene, kSilhouette, method, q, stream);
+}
+
+int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int64_t *n_bytes) {
+    if (!st || !n_bytes) return fail(GQ_ERR_INVALID, "null argument");
+    WideHost wh;
+    try {
+        build_wide(st->h, wh);
+    } catch (const std::bad_alloc &) {
+        return fail(GQ_ERR_OOM, "host allocation failed");
+    }
+    const void *src = nullptr;
+    size_t bytes = 0;
+    uint32_t scal[5] = {wh.root_ref, wh.sroot_ref, (uint32_t)wh.n_nodes, (uint32_t)wh.n_snch_nodes, (uint32_t)wh.wide_depth};
+    switch (which) {
+    case 0: src = wh.qnodes.data(); bytes = wh.qnodes.size(); break;
+    case 1: src = wh.nodes.data(); bytes = wh.nodes.size(); break;
+    case 2: src = wh.qsnodes.data(); bytes = wh.qsnodes.size(); break;
+    case 3: src = wh.snodes.data(); bytes = wh.snodes.size(); break;
+    case 4: src = wh.prims.data(); bytes = wh.prims.size() * sizeof(float4); break;
+    case 5: src = wh.prim_ids.data(); bytes = wh.prim_ids.size() * 4; break;
+    case 6: src = wh.sils.data(); bytes = wh.sils.size() * sizeof(float4); break;
+    case 7: src = wh.sil_ids.data(); bytes = wh.sil_ids.size() * 4; break;
+    case 8: src = wh.ref_cones.data(); bytes = wh.ref_cones.size() * sizeof(RefCone); break;
+    case 9: src = wh.big_leaves.data(); bytes = wh.big_leaves.size() * 4; break;
+    case 10: src = scal; bytes = sizeof scal; break;
+    default: return fail(GQ_ERR_INVALID, "unknown layout array");
+    }
+    if (out) {
+        if (*n_bytes < (int64_t)bytes) return fail(GQ_ERR_INVALID, "output buffer too small");
+        if (bytes) std::memcpy(out, src, bytes);
+    }
+    *n_bytes = (int64_t)bytes;
+    return GQ_OK;
 }
 
 int gq_profile_begin(void) {

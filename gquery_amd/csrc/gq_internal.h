@@ -21,18 +21,25 @@ struct PlainView {
 };
 
 // Engine layout (built on the host by wide.cpp, uploaded by the API layer).
+struct RefCone;
 struct WideView {
     int dim;
-    int width;                // children per node (4 or 8)
-    int64_t n_nodes, n_snch_nodes;
-    const void *nodes;        // fp32 wide nodes
-    const void *qnodes;       // quantized wide nodes
-    const void *snodes;       // fp32 wide SNCH nodes (boxes + cones)
-    const void *qsnodes;      // quantized wide SNCH nodes
-    const float4 *prims;      // 3D: 3 float4 per triangle (a|id, b, c); 2D: 1 float4 per segment
+    int width;                // children per node (4)
+    int wide_depth;           // deepest BVH4 node (root = 0)
+    int bin_depth;            // deepest binary node
+    int64_t n_nodes, n_snch_nodes, n_prims, n_sils;
+    const void *nodes;        // FNode<D>[n_nodes]  float32 BVH4
+    const void *qnodes;       // QNode<D>[n_nodes]  quantized BVH4
+    const void *snodes;       // SPairF<D>[n_snch_nodes]
+    const void *qsnodes;      // SPairQ<D>[n_snch_nodes]
+    const float4 *prims;      // 3D: 3 float4 per triangle (a, b, c); 2D: 1 float4 per segment (a|b)
     const int32_t *prim_ids;  // original id per ordered slot
-    const float4 *sils;       // engine silhouette records
+    const float4 *sils;       // 3D: 3 float4 per silhouette (a, b, n0, n1); 2D: 2 float4 (a|b, c)
     const int32_t *sil_ids;
+    const RefCone *ref_cones;
+    const uint32_t *big_leaves;
+    uint32_t root_ref, sroot_ref;
+    float sroot_lo[3], sroot_hi[3];
     float scene_lo[3], scene_hi[3];
 };
 

@@ -1,0 +1,430 @@
+// Host construction of the engine layout (see wide.h).
+//
+// Quantization: a child box coordinate is stored as q with decode(q) = fma(q, 2^e, origin), the
+// exact operation the kernels perform; q is chosen (and verified with that same decode) so the
+// decoded box contains the true child box.  Cones of the exact cull are computed in double from
+// the float32 silhouette records and widened by a fixed slack before rounding.
+#include "wide.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+namespace gqa {
+namespace {
+
+constexpr double kConeSlack = 1e-5;   // radians added to every exact-cull cone
+constexpr double kCullMaxAngle = 1.45; // cones wider than this never cull (cos_t <= 0)
+
+struct BinNode {
+    float lo[3], hi[3];
+    bool leaf;
+    uint32_t off, cnt, second;  // leaf: prim offset/count; interior: right child offset
+    uint32_t pbegin, pcount;    // subtree primitive range (contiguous in leaf order)
+};
+
+template <int D> std::vector<BinNode> read_bvh(const HostStructure &h) {
+    const int ns = 2 * D + 3;
+    std::vector<BinNode> b((size_t)h.n_nodes);
+    for (size_t i = 0; i < b.size(); ++i) {
+        const float *n = h.bvh_node_data.data() + i * ns;
+        BinNode &x = b[i];
+        for (int k = 0; k < 3; ++k) { x.lo[k] = k < D ? n[k] : 0.0f; x.hi[k] = k < D ? n[D + k] : 0.0f; }
+        x.cnt = (uint32_t)n[2 * D + 2];
+        x.leaf = x.cnt > 0;
+        x.off = (uint32_t)n[2 * D];
+        x.second = (uint32_t)n[2 * D + 1];
+    }
+    for (size_t i = b.size(); i-- > 0;) {
+        BinNode &x = b[i];
+        if (x.leaf) { x.pbegin = x.off; x.pcount = x.cnt; }
+        else {
+            const BinNode &l = b[i + 1], &r = b[i + x.second];
+            x.pbegin = l.pbegin;
+            x.pcount = l.pcount + r.pcount;
+        }
+    }
+    return b;
+}
+
+inline float decode(int q, int e, float origin) { return std::fmaf((float)q, std::ldexp(1.0f, e), origin); }
+
+// Per-axis quantization frame of a box: exponent so that decode(levels) >= hi.
+inline int frame_exp(float lo, float hi, int levels) {
+    double ext = (double)hi - (double)lo;
+    if (!(ext > 0)) return -100;
+    int e = (int)std::ceil(std::log2(ext / levels));
+    e = std::max(e, -100);
+    while (decode(levels, e, lo) < hi) ++e;
+    while (e > -100 && decode(levels, e - 1, lo) >= hi) --e;
+    return e;
+}
+
+inline void quantize(float clo, float chi, float origin, int e, int levels, int &qlo, int &qhi) {
+    double s = std::ldexp(1.0, e);
+    qlo = (int)std::floor(((double)clo - origin) / s);
+    qhi = (int)std::ceil(((double)chi - origin) / s);
+    qlo = std::clamp(qlo, 0, levels);
+    qhi = std::clamp(qhi, 0, levels);
+    while (qlo > 0 && decode(qlo, e, origin) > clo) --qlo;
+    while (qlo < levels && decode(qlo + 1, e, origin) <= clo) ++qlo;
+    while (qhi < levels && decode(qhi, e, origin) < chi) ++qhi;
+    while (qhi > qlo && decode(qhi - 1, e, origin) >= chi) --qhi;
+}
+
+// Leaf references: counts above 126 go through a side table (count field 127).
+struct LeafTable {
+    std::vector<uint32_t> big;  // pairs (first, count)
+    uint32_t ref(uint32_t first, uint32_t count) {
+        if (count <= 126) return make_leaf_ref(first, count);
+        uint32_t idx = (uint32_t)(big.size() / 2);
+        big.push_back(first);
+        big.push_back(count);
+        return make_leaf_ref(idx, 127);
+    }
+};
+
+template <int D>
+void build_bvh4(const std::vector<BinNode> &b, int max_leaf, WideHost &out, LeafTable &lt) {
+    struct Tmp {
+        int n = 0;
+        int bin[4];
+        uint32_t ref[4];
+    };
+    std::vector<Tmp> tmp;
+    auto area = [&](int i) {
+        const BinNode &x = b[i];
+        double e[3];
+        for (int k = 0; k < 3; ++k) e[k] = k < D ? (double)x.hi[k] - x.lo[k] : 0.0;
+        return D == 2 ? e[0] + e[1] : e[0] * e[1] + e[0] * e[2] + e[1] * e[2];
+    };
+    auto is_leafish = [&](int i) { return b[i].leaf || (int)b[i].pcount <= max_leaf; };
+    std::function<uint32_t(int, int)> make = [&](int bi, int depth) -> uint32_t {
+        out.wide_depth = std::max(out.wide_depth, depth);
+        uint32_t idx = (uint32_t)tmp.size();
+        tmp.emplace_back();
+        int list[4], n = 0;
+        if (is_leafish(bi)) {
+            list[n++] = bi;  // only for the root of a tiny tree
+        } else {
+            list[n++] = bi + 1;
+            list[n++] = bi + (int)b[bi].second;
+            while (n < 4) {
+                int best = -1;
+                double ba = -1;
+                for (int j = 0; j < n; ++j)
+                    if (!is_leafish(list[j]) && area(list[j]) > ba) { ba = area(list[j]); best = j; }
+                if (best < 0) break;
+                int c = list[best];
+                list[best] = c + 1;
+                list[n++] = c + (int)b[c].second;
+            }
+        }
+        Tmp t;
+        t.n = n;
+        for (int j = 0; j < n; ++j) {
+            t.bin[j] = list[j];
+            if (is_leafish(list[j])) t.ref[j] = lt.ref(b[list[j]].pbegin, b[list[j]].pcount);
+            else t.ref[j] = make(list[j], depth + 1);
+        }
+        tmp[idx] = t;
+        return idx;
+    };
+    make(0, 0);
+    const size_t n = tmp.size();
+    out.n_nodes = n;
+    out.nodes.assign(n * sizeof(FNode<D>), 0);
+    out.qnodes.assign(n * sizeof(QNode<D>), 0);
+    constexpr int L = QBits<D>::levels;
+    for (size_t i = 0; i < n; ++i) {
+        const Tmp &t = tmp[i];
+        FNode<D> fn;
+        std::memset(&fn, 0, sizeof fn);
+        QNode<D> qn;
+        std::memset(&qn, 0, sizeof qn);
+        fn.nchild = (uint32_t)t.n;
+        qn.nchild = (uint8_t)t.n;
+        // node frame = union of the children
+        float nlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, nhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int j = 0; j < t.n; ++j)
+            for (int k = 0; k < D; ++k) {
+                nlo[k] = std::min(nlo[k], b[t.bin[j]].lo[k]);
+                nhi[k] = std::max(nhi[k], b[t.bin[j]].hi[k]);
+            }
+        int e[3];
+        for (int k = 0; k < D; ++k) {
+            e[k] = frame_exp(nlo[k], nhi[k], L);
+            qn.origin[k] = nlo[k];
+            qn.exp[k] = (int8_t)std::clamp(e[k], -127, 127);
+            e[k] = qn.exp[k];
+        }
+        for (int j = 0; j < 4; ++j) {
+            if (j < t.n) {
+                const BinNode &c = b[t.bin[j]];
+                fn.child[j] = qn.child[j] = t.ref[j];
+                for (int k = 0; k < D; ++k) {
+                    fn.lo[k][j] = c.lo[k];
+                    fn.hi[k][j] = c.hi[k];
+                    int ql, qh;
+                    quantize(c.lo[k], c.hi[k], nlo[k], e[k], L, ql, qh);
+                    qn.qlo[k][j] = (typename QBits<D>::T)ql;
+                    qn.qhi[k][j] = (typename QBits<D>::T)qh;
+                }
+            } else {
+                // empty slot: an inverted box that no query overlaps
+                for (int k = 0; k < D; ++k) {
+                    fn.lo[k][j] = FLT_MAX;
+                    fn.hi[k][j] = -FLT_MAX;
+                    qn.qlo[k][j] = (typename QBits<D>::T)L;
+                    qn.qhi[k][j] = 0;
+                }
+                fn.child[j] = qn.child[j] = 0;
+            }
+        }
+        std::memcpy(out.nodes.data() + i * sizeof(FNode<D>), &fn, sizeof fn);
+        std::memcpy(out.qnodes.data() + i * sizeof(QNode<D>), &qn, sizeof qn);
+    }
+}
+
+// float32 predicate normals exactly as the traversal computes them: normalize(cross(u, v)) with
+// the fma cross product and v * (1 / sqrt(dot(v, v))).
+inline void fcross(const float *a, const float *b, float *o) {
+    o[0] = std::fmaf(a[1], b[2], -(a[2] * b[1]));
+    o[1] = std::fmaf(a[2], b[0], -(a[0] * b[2]));
+    o[2] = std::fmaf(a[0], b[1], -(a[1] * b[0]));
+}
+inline void fnormalize(float *v) {
+    float d = std::fmaf(v[2], v[2], std::fmaf(v[1], v[1], v[0] * v[0]));
+    float s = 1.0f / std::sqrt(d);
+    for (int k = 0; k < 3; ++k) v[k] = v[k] * s;
+}
+
+struct SilNormals {
+    double n0[3], n1[3];
+    bool ok;
+};
+
+template <int D>
+void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, WideHost &out, LeafTable &lt) {
+    const int nss = 3 * D + 7, ss = kSilStride[D];
+    const size_t nn = b.size();
+    const float *snd = h.snch_node_data.data();
+    // silhouette records in leaf order
+    const size_t ns = (size_t)h.n_silhouettes;
+    std::vector<SilNormals> nrm(ns);
+    if (D == 3) {
+        out.sils.resize(ns * 3);
+        for (size_t s = 0; s < ns; ++s) {
+            const float *r = h.silhouette_data.data() + s * ss;
+            const float *c = r, *a = r + 3, *bb = r + 6, *e = r + 9;
+            float ba[3], ca[3], ea[3], n0[3], n1[3];
+            for (int k = 0; k < 3; ++k) { ba[k] = bb[k] - a[k]; ca[k] = c[k] - a[k]; ea[k] = e[k] - a[k]; }
+            fcross(ba, ca, n0);
+            fnormalize(n0);
+            fcross(ea, ba, n1);
+            fnormalize(n1);
+            out.sils[3 * s + 0] = make_float4(a[0], a[1], a[2], bb[0]);
+            out.sils[3 * s + 1] = make_float4(bb[1], bb[2], n0[0], n0[1]);
+            out.sils[3 * s + 2] = make_float4(n0[2], n1[0], n1[1], n1[2]);
+            // exact-cull cones bound the float32 normals the predicate actually uses; a non-finite
+            // normal makes the predicate NaN, i.e. never a silhouette
+            SilNormals &sn = nrm[s];
+            sn.ok = true;
+            for (int k = 0; k < 3; ++k) {
+                sn.n0[k] = n0[k];
+                sn.n1[k] = n1[k];
+                sn.ok = sn.ok && std::isfinite(n0[k]) && std::isfinite(n1[k]);
+            }
+            out.sil_ids.push_back((int32_t)r[18]);
+        }
+    } else {
+        out.sils.resize(ns * 2);
+        for (size_t s = 0; s < ns; ++s) {
+            const float *r = h.silhouette_data.data() + s * ss;
+            out.sils[2 * s + 0] = make_float4(r[0], r[1], r[2], r[3]);
+            out.sils[2 * s + 1] = make_float4(r[4], r[5], 0.0f, 0.0f);
+            out.sil_ids.push_back((int32_t)r[9]);
+            nrm[s].ok = true;
+        }
+    }
+    // subtree silhouette ranges (contiguous in leaf order)
+    std::vector<uint32_t> sb(nn), sc(nn);
+    for (size_t i = nn; i-- > 0;) {
+        const float *n = snd + i * nss;
+        if (b[i].leaf) { sb[i] = (uint32_t)n[3 * D + 5]; sc[i] = (uint32_t)n[3 * D + 6]; }
+        else {
+            size_t l = i + 1, r = i + b[i].second;
+            sb[i] = sb[l];
+            sc[i] = (sb[r] + sc[r]) - sb[l];
+        }
+    }
+    // exact-cull cone of a subtree
+    auto cone_of = [&](size_t i, ConeF &cf) {
+        cf.axis[0] = 0; cf.axis[1] = 0; cf.axis[2] = 1;
+        cf.cos_t = 0.0f; cf.sin_t = 1.0f;  // never culls
+        bool any = false;
+        for (uint32_t s = sb[i]; s < sb[i] + sc[i] && !any; ++s) any = nrm[s].ok;
+        if (!any) { cf.cos_t = 2.0f; cf.sin_t = 0.0f; return; }  // no possible silhouette below: always cull
+        if (D != 3) return;
+        // orient each (n0, n1) pair along a common direction (the predicate is invariant under
+        // flipping both), then bound all of them by one cone
+        bool have_ref = false;
+        double r[3] = {0, 0, 0}, sum[3] = {0, 0, 0};
+        for (uint32_t s = sb[i]; s < sb[i] + sc[i]; ++s) {
+            const SilNormals &q = nrm[s];
+            if (!q.ok) continue;
+            double m[3] = {q.n0[0] + q.n1[0], q.n0[1] + q.n1[1], q.n0[2] + q.n1[2]};
+            if (!have_ref) { std::memcpy(r, m, sizeof r); have_ref = true; }
+            double sg = (m[0] * r[0] + m[1] * r[1] + m[2] * r[2]) >= 0 ? 1.0 : -1.0;
+            for (int k = 0; k < 3; ++k) sum[k] += sg * m[k];
+        }
+        double l = std::sqrt(sum[0] * sum[0] + sum[1] * sum[1] + sum[2] * sum[2]);
+        if (!(l > 0)) return;
+        double ax[3] = {sum[0] / l, sum[1] / l, sum[2] / l};
+        double cmin = 1.0;
+        for (uint32_t s = sb[i]; s < sb[i] + sc[i]; ++s) {
+            const SilNormals &q = nrm[s];
+            if (!q.ok) continue;
+            double l0 = std::sqrt(q.n0[0] * q.n0[0] + q.n0[1] * q.n0[1] + q.n0[2] * q.n0[2]);
+            double l1 = std::sqrt(q.n1[0] * q.n1[0] + q.n1[1] * q.n1[1] + q.n1[2] * q.n1[2]);
+            if (!(l0 > 0) || !(l1 > 0)) return;  // a zero normal: the dot is 0, never provably signed
+            double m = (q.n0[0] + q.n1[0]) * ax[0] + (q.n0[1] + q.n1[1]) * ax[1] + (q.n0[2] + q.n1[2]) * ax[2];
+            double sg = m >= 0 ? 1.0 : -1.0;
+            double c0 = sg * (q.n0[0] * ax[0] + q.n0[1] * ax[1] + q.n0[2] * ax[2]) / l0;
+            double c1 = sg * (q.n1[0] * ax[0] + q.n1[1] * ax[1] + q.n1[2] * ax[2]) / l1;
+            cmin = std::min(cmin, std::min(c0, c1));
+        }
+        double th = std::acos(std::clamp(cmin, -1.0, 1.0)) + kConeSlack;
+        if (th >= kCullMaxAngle) return;
+        for (int k = 0; k < 3; ++k) cf.axis[k] = (float)ax[k];
+        // the float axis differs from ax by < 1e-7 rad, far inside kConeSlack
+        cf.cos_t = std::nextafter((float)std::cos(th), -1.0f);
+        cf.sin_t = std::nextafter((float)std::sin(th), 2.0f);
+    };
+    auto child_ref = [&](size_t c, const std::vector<uint32_t> &pidx) -> uint32_t {
+        if (b[c].leaf) return lt.ref(sb[c], sc[c]);
+        return pidx[c];
+    };
+    // pair index of every interior node, depth-first
+    std::vector<uint32_t> pidx(nn, 0xFFFFFFFFu);
+    uint32_t np = 0;
+    for (size_t i = 0; i < nn; ++i)
+        if (!b[i].leaf) pidx[i] = np++;
+    out.n_snch_nodes = np;
+    out.snodes.assign((size_t)std::max<uint32_t>(np, 1) * sizeof(SPairF<D>), 0);
+    out.qsnodes.assign((size_t)std::max<uint32_t>(np, 1) * sizeof(SPairQ<D>), 0);
+    constexpr int L = QBits<D>::levels;
+    const float kHalfPi = 1.57079637050628662109375f;
+    for (size_t i = 0; i < nn; ++i) {
+        if (b[i].leaf) continue;
+        const size_t ch[2] = {i + 1, i + b[i].second};
+        SPairF<D> pf;
+        std::memset(&pf, 0, sizeof pf);
+        SPairQ<D> pq;
+        std::memset(&pq, 0, sizeof pq);
+        int e[3];
+        for (int k = 0; k < D; ++k) {
+            float lo = std::min(b[ch[0]].lo[k], b[ch[1]].lo[k]);
+            float hi = std::max(b[ch[0]].hi[k], b[ch[1]].hi[k]);
+            e[k] = std::clamp(frame_exp(lo, hi, L), -127, 127);
+            pq.origin[k] = lo;
+            pq.exp[k] = (int8_t)e[k];
+        }
+        for (int j = 0; j < 2; ++j) {
+            const size_t c = ch[j];
+            const BinNode &x = b[c];
+            for (int k = 0; k < D; ++k) {
+                pf.lo[j][k] = x.lo[k];
+                pf.hi[j][k] = x.hi[k];
+                int ql, qh;
+                quantize(x.lo[k], x.hi[k], pq.origin[k], e[k], L, ql, qh);
+                pq.qlo[j][k] = (typename QBits<D>::T)ql;
+                pq.qhi[j][k] = (typename QBits<D>::T)qh;
+            }
+            pf.child[j] = pq.child[j] = child_ref(c, pidx);
+            // the reference cone test can only cull when 0 <= half <= pi/2 (a wider cone always
+            // contains a direction perpendicular to every view direction; a negative one is unset)
+            const float *n = snd + c * nss;
+            const float half = n[3 * D];
+            if (half >= 0.0f && !(half > kHalfPi)) {
+                RefCone rc;
+                std::memset(&rc, 0, sizeof rc);
+                for (int k = 0; k < D; ++k) { rc.axis[k] = n[2 * D + k]; rc.lo[k] = n[k]; rc.hi[k] = n[D + k]; }
+                rc.half = half;
+                rc.radius = n[3 * D + 1];
+                pf.ref_cone[j] = pq.ref_cone[j] = (int32_t)out.ref_cones.size();
+                out.ref_cones.push_back(rc);
+            } else {
+                pf.ref_cone[j] = pq.ref_cone[j] = -1;
+            }
+            ConeF cf;
+            cone_of(c, cf);
+            pf.cone[j] = cf;
+            // int16 cone: the axis rounding error is folded into the angle
+            double ax[3] = {cf.axis[0], cf.axis[1], cf.axis[2]};
+            int16_t qa[3];
+            for (int k = 0; k < 3; ++k) qa[k] = (int16_t)std::lround(std::clamp(ax[k], -1.0, 1.0) * 32767.0);
+            double dq[3] = {qa[0] / 32767.0, qa[1] / 32767.0, qa[2] / 32767.0};
+            double lq = std::sqrt(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
+            for (int k = 0; k < 3; ++k) pq.cone_axis[j][k] = qa[k];
+            if (cf.cos_t > 1.5f) {
+                pq.cone_cs[j][0] = 32767;   // "always cull" (no silhouettes below)
+                pq.cone_cs[j][1] = -32767;
+            } else if (cf.cos_t <= 0.0f || !(lq > 0)) {
+                pq.cone_cs[j][0] = 0;
+                pq.cone_cs[j][1] = 32767;
+            } else {
+                double dot = (ax[0] * dq[0] + ax[1] * dq[1] + ax[2] * dq[2]) / lq;
+                double err = std::acos(std::clamp(dot, -1.0, 1.0)) + 2.0 / 32767.0;
+                double th = std::acos(std::clamp((double)cf.cos_t, -1.0, 1.0)) + err;
+                if (th >= kCullMaxAngle) { pq.cone_cs[j][0] = 0; pq.cone_cs[j][1] = 32767; }
+                else {
+                    pq.cone_cs[j][0] = (int16_t)std::floor(std::cos(th) * 32767.0);
+                    pq.cone_cs[j][1] = (int16_t)std::min(32767.0, std::ceil(std::sin(th) * 32767.0));
+                }
+            }
+        }
+        std::memcpy(out.snodes.data() + (size_t)pidx[i] * sizeof(SPairF<D>), &pf, sizeof pf);
+        std::memcpy(out.qsnodes.data() + (size_t)pidx[i] * sizeof(SPairQ<D>), &pq, sizeof pq);
+    }
+    out.sroot_ref = b[0].leaf ? lt.ref(sb[0], sc[0]) : 0u;
+    for (int k = 0; k < D; ++k) { out.sroot_lo[k] = b[0].lo[k]; out.sroot_hi[k] = b[0].hi[k]; }
+}
+
+template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_leaf) {
+    std::vector<BinNode> b = read_bvh<D>(h);
+    LeafTable lt;
+    build_bvh4<D>(b, max_leaf, out, lt);
+    out.root_ref = 0;
+    for (int k = 0; k < D; ++k) { out.scene_lo[k] = b[0].lo[k]; out.scene_hi[k] = b[0].hi[k]; }
+    // primitives in leaf order
+    const int ps = kPrimStride[D];
+    out.prims.resize((size_t)h.n_prims * (D == 3 ? 3 : 1));
+    out.prim_ids.resize((size_t)h.n_prims);
+    for (int64_t s = 0; s < h.n_prims; ++s) {
+        const float *r = h.primitive_data.data() + s * ps;
+        if (D == 3) {
+            out.prims[3 * s + 0] = make_float4(r[0], r[1], r[2], 0.0f);
+            out.prims[3 * s + 1] = make_float4(r[3], r[4], r[5], 0.0f);
+            out.prims[3 * s + 2] = make_float4(r[6], r[7], r[8], 0.0f);
+        } else {
+            out.prims[s] = make_float4(r[0], r[1], r[2], r[3]);
+        }
+        out.prim_ids[s] = (int32_t)r[D * D];
+    }
+    if (h.is_snch) build_snch_pairs<D>(h, b, out, lt);
+    out.big_leaves = lt.big;
+}
+
+} // namespace
+
+void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims) {
+    out = WideHost();
+    max_leaf_prims = std::clamp(max_leaf_prims, 1, 16);
+    if (h.dim == 2) build_impl<2>(h, out, max_leaf_prims);
+    else build_impl<3>(h, out, max_leaf_prims);
+}
+
+} // namespace gqa

@@ -1,6 +1,18 @@
-// Engine layout: wide (collapsed) BVH / SNCH nodes in 64-byte records, fp32 and quantized
-// variants, primitives and silhouettes in leaf order as float4 records; plus the per-scene
-// workspace the engine launcher keeps between calls.
+// Engine layout, built on the host from the reference-layout BVH / SNCH:
+//
+//  * BVH4: the binary SAH tree collapsed to 4-wide nodes (each node's children are the up to four
+//    largest-area descendants of a binary node), small subtrees folded into leaves.  Two formats,
+//    both one or two 64-byte cache lines, structure-of-arrays over the 4 children:
+//      QNode<D>  64 B: child boxes quantized to 8 bits (3D) / 16 bits (2D) relative to the node's
+//                      box, rounded outward so every decoded child box contains the true one;
+//      FNode<D> 128 B: child boxes in float32.
+//  * SNCH pairs: one record per binary interior node holding both children (box, reference
+//    normal-cone index, exact-cull cone, child reference) so a visit is one record fetch.
+//      SPairQ<D> 64 B (quantized boxes / cones), SPairF<D> 128 B (float32).
+//  * primitives in leaf order as float4 records (3D: a, b, c; 2D: a|b), silhouettes in leaf
+//    order (3D: a, b, n0, n1 with the predicate normals precomputed; 2D: a|b, c).
+// Child references: bit 31 set = leaf, bits 24..30 = count, bits 0..23 = first primitive /
+// silhouette; otherwise the index of the child node / pair record.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,21 +25,120 @@
 
 namespace gqa {
 
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr int kMaxLeafPrims = 127;
+__host__ __device__ inline bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0; }
+__host__ __device__ inline uint32_t ref_count(uint32_t r) { return (r >> 24) & 0x7Fu; }
+__host__ __device__ inline uint32_t ref_first(uint32_t r) { return r & 0xFFFFFFu; }
+__host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+    return kLeafBit | (count << 24) | first;
+}
+
+template <int D> struct QBits;                 // quantized coordinate type per dimension
+template <> struct QBits<3> { using T = uint8_t; static constexpr int levels = 255; };
+template <> struct QBits<2> { using T = uint16_t; static constexpr int levels = 65535; };
+
+// 4-wide BVH node, quantized child boxes.
+template <int D> struct QNode;
+template <> struct alignas(64) QNode<3> {
+    float origin[3];
+    int8_t exp[3];          // per-axis scale 2^exp
+    uint8_t nchild;
+    uint8_t qlo[3][4], qhi[3][4];
+    uint32_t child[4];
+    uint8_t pad[8];
+};
+template <> struct alignas(64) QNode<2> {
+    float origin[2];
+    int8_t exp[2];
+    uint8_t nchild, pad0;
+    uint16_t qlo[2][4], qhi[2][4];
+    uint32_t child[4];
+    uint8_t pad[4];
+};
+static_assert(sizeof(QNode<3>) == 64, "QNode<3> must be one 64-byte line");
+static_assert(sizeof(QNode<2>) == 64, "QNode<2> must be one 64-byte line");
+
+// 4-wide BVH node, float32 child boxes.
+template <int D> struct alignas(64) FNode {
+    float lo[D][4], hi[D][4];
+    uint32_t child[4];
+    uint32_t nchild;
+    uint8_t pad[128 - (2 * D * 16 + 20)];
+};
+static_assert(sizeof(FNode<3>) == 128 && sizeof(FNode<2>) == 128, "FNode is two lines");
+
+// Exact-cull cone of an SNCH child: every predicate normal (oriented consistently) lies within
+// angle acos(cos_t) of `axis`; cos_t/sin_t are rounded so the cone is conservative.  cos_t <= 0
+// marks a cone that never culls.
+struct ConeF {
+    float axis[3];
+    float cos_t, sin_t;
+};
+
+// SNCH pair record (both children of one binary interior node), float32.
+template <int D> struct alignas(64) SPairF {
+    float lo[2][D], hi[2][D];
+    ConeF cone[2];
+    uint32_t child[2];
+    int32_t ref_cone[2];    // index into the reference-cone table, -1 = the reference test cannot cull
+    uint8_t pad[128 - (4 * D * 4 + 2 * 20 + 16)];
+};
+static_assert(sizeof(SPairF<3>) == 128 && sizeof(SPairF<2>) == 128, "SPairF is two lines");
+
+// SNCH pair record, quantized boxes (8-bit 3D / 16-bit 2D) and int16 cones.
+template <int D> struct SPairQ;
+template <> struct alignas(64) SPairQ<3> {
+    float origin[3];
+    int8_t exp[3];
+    uint8_t pad0;
+    uint8_t qlo[2][3], qhi[2][3];
+    int16_t cone_axis[2][3];   // unit axis * 32767 (rounded; the error is added to the angle)
+    int16_t cone_cs[2][2];     // cos_t (rounded down), sin_t (rounded up), * 32767
+    uint32_t child[2];
+    int32_t ref_cone[2];
+};
+template <> struct alignas(64) SPairQ<2> {
+    float origin[2];
+    int8_t exp[2];
+    uint8_t pad0[2];
+    uint16_t qlo[2][2], qhi[2][2];
+    int16_t cone_axis[2][3];
+    int16_t cone_cs[2][2];
+    uint32_t child[2];
+    int32_t ref_cone[2];
+};
+static_assert(sizeof(SPairQ<3>) == 64 && sizeof(SPairQ<2>) == 64, "SPairQ is one line");
+
+// Reference normal cone (snch.cpp:102-154 values, float32) of a node whose cone test can cull
+// (0 <= half_angle <= pi/2); tested exactly as SNCHNode.overlaps does.
+struct RefCone {
+    float axis[3];
+    float half, radius;
+    float lo[3], hi[3];   // the node's box (the reference test uses its centroid and p_max)
+};
+
 struct WideHost {
     int width = 4;
+    int wide_depth = 0;
     size_t n_nodes = 0, n_snch_nodes = 0;
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
-    std::vector<uint8_t> nodes, qnodes, snodes, qsnodes;  // raw 64-byte records
+    uint32_t root_ref = 0;       // BVH4 root (node 0, or a leaf ref for a one-leaf tree)
+    uint32_t sroot_ref = 0;      // SNCH root pair (0) or leaf ref
+    float sroot_lo[3] = {0, 0, 0}, sroot_hi[3] = {0, 0, 0};
+    std::vector<uint8_t> nodes, qnodes, snodes, qsnodes;  // raw records
     std::vector<float4> prims;
     std::vector<int32_t> prim_ids;
     std::vector<float4> sils;
     std::vector<int32_t> sil_ids;
+    std::vector<RefCone> ref_cones;
+    std::vector<uint32_t> big_leaves;  // (first, count) pairs of leaves with more than 126 entries
 };
 
-void build_wide(const HostStructure &h, WideHost &out);
+void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims = 4);
 
-// Device scratch owned by a scene: query keys / permutation for the Morton sort.  Grown on
-// demand (hipMalloc) and reused by later calls on the same stream order.
+// Device scratch owned by a scene (Morton keys, permutation, radix-sort temp).  Grown on demand
+// with hipMalloc; reused by later calls (calls on one scene are ordered on their streams).
 struct EngineWorkspace {
     void *buf = nullptr;
     size_t bytes = 0;
@@ -37,5 +148,7 @@ struct EngineWorkspace {
 
 hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q,
                          EngineWorkspace &ws, hipStream_t st);
+// Whether the engine's fixed-capacity traversal stack covers this scene's tree depth.
+bool engine_supported(QueryKind kind, const WideView &w);
 
 } // namespace gqa

@@ -78,6 +78,16 @@ class _Structure:
     def primitive_data(self) -> np.ndarray:
         return self._fetch("gq_structure_primitive_data")
 
+    def engine_layout(self, which: int) -> np.ndarray:
+        """Raw bytes of one engine-layout array built on the host (see gq_structure_engine_layout)."""
+        L = _lib.lib()
+        n = C.c_int64(0)
+        _lib.check(L.gq_structure_engine_layout(self._h, which, None, C.byref(n)), "engine_layout")
+        out = np.empty(n.value, dtype=np.uint8)
+        if n.value:
+            _lib.check(L.gq_structure_engine_layout(self._h, which, out.ctypes.data, C.byref(n)), "engine_layout")
+        return out
+
 
 class _SnchMixin:
     def silhouette_data(self) -> np.ndarray:

@@ -112,6 +112,15 @@ int gq_intersect_bvh(gq_scene *scene, int method, const float *x, const float *v
 int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, const float *r_max,
                                float r_max_scalar, int64_t n, const gq_records *out, void *stream);
 
+/* ---- Engine layout inspection (no counterpart in the reference: exposes the host-built engine
+ * arrays so tests can check them on machines without a GPU) ---- */
+
+/* which: 0 quantized BVH4 nodes, 1 float32 BVH4 nodes, 2 quantized SNCH pairs, 3 float32 SNCH
+ * pairs, 4 primitives (float4), 5 primitive ids, 6 silhouettes (float4), 7 silhouette ids,
+ * 8 reference cones, 9 big-leaf table, 10 scalars (root_ref, sroot_ref, n_nodes, n_pairs,
+ * wide_depth as uint32).  Pass out = NULL to get the byte count in *n_bytes. */
+int gq_structure_engine_layout(const gq_structure *s, int which, void *out, int64_t *n_bytes);
+
 /* ---- Kernel timing (replaces the time.time() brackets around dr.eval in the reference demos,
  * demo/closest_point_demo_3d.py:103-107) ---- */
 

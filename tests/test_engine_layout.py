@@ -1,0 +1,186 @@
+"""Engine layout built on the host (CPU-only checks): every reference in range, every primitive and
+silhouette reachable exactly once, decoded quantized boxes containing the true geometry, and the
+exact-cull cones containing every predicate normal below them."""
+import numpy as np
+import pytest
+
+from gquery_amd import ext, synth
+
+LEAF = 0x80000000
+
+
+def qnode_dtype(D):
+    if D == 3:
+        return np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("nchild", "u1"), ("qlo", "u1", (3, 4)),
+                         ("qhi", "u1", (3, 4)), ("child", "<u4", 4), ("pad", "u1", 8)])
+    return np.dtype([("origin", "<f4", 2), ("exp", "i1", 2), ("nchild", "u1"), ("pad0", "u1"),
+                     ("qlo", "<u2", (2, 4)), ("qhi", "<u2", (2, 4)), ("child", "<u4", 4), ("pad", "u1", 4)])
+
+
+def fnode_dtype(D):
+    return np.dtype([("lo", "<f4", (D, 4)), ("hi", "<f4", (D, 4)), ("child", "<u4", 4), ("nchild", "<u4"),
+                     ("pad", "u1", 128 - (2 * D * 16 + 20))])
+
+
+def spq_dtype(D):
+    if D == 3:
+        return np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("pad0", "u1"), ("qlo", "u1", (2, 3)),
+                         ("qhi", "u1", (2, 3)), ("axis", "<i2", (2, 3)), ("cs", "<i2", (2, 2)),
+                         ("child", "<u4", 2), ("ref", "<i4", 2)])
+    return np.dtype([("origin", "<f4", 2), ("exp", "i1", 2), ("pad0", "u1", 2), ("qlo", "<u2", (2, 2)),
+                     ("qhi", "<u2", (2, 2)), ("axis", "<i2", (2, 3)), ("cs", "<i2", (2, 2)),
+                     ("child", "<u4", 2), ("ref", "<i4", 2)])
+
+
+def spf_dtype(D):
+    return np.dtype([("lo", "<f4", (2, D)), ("hi", "<f4", (2, D)), ("axis0", "<f4", 3), ("cos0", "<f4"),
+                     ("sin0", "<f4"), ("axis1", "<f4", 3), ("cos1", "<f4"), ("sin1", "<f4"),
+                     ("child", "<u4", 2), ("ref", "<i4", 2), ("pad", "u1", 128 - (4 * D * 4 + 56))])
+
+
+def decode(q, e, origin):
+    """fma(q, 2^e, origin) rounded once to float32 (exact in float64 for these magnitudes)."""
+    return (q.astype(np.float64) * np.ldexp(1.0, e.astype(np.int64)) + origin.astype(np.float64)).astype(np.float32)
+
+
+def leaf_range(ref, big):
+    first, count = ref & 0xFFFFFF, (ref >> 24) & 0x7F
+    if count == 127:
+        return int(big[2 * first]), int(big[2 * first + 1])
+    return int(first), int(count)
+
+
+SCENES = {
+    "star300": lambda: synth.star_polyline(300, seed=3),
+    "star8192": lambda: synth.star_polyline(8192, seed=4),
+    "ico1": lambda: synth.displaced_icosphere(1),
+    "ico4": lambda: synth.displaced_icosphere(4, seed=6),
+    "dup3d": lambda: (np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (300, 1)),
+                      np.arange(900, dtype=np.int32).reshape(300, 3)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_bvh4_layout(name):
+    V, F = SCENES[name]()
+    D = V.shape[1]
+    s = (ext.SNCH if D == 2 else ext.SNCH3D)(V, F)
+    scal = s.engine_layout(10).view(np.uint32)
+    root, n_nodes = scal[0], scal[2]
+    q = s.engine_layout(0).view(qnode_dtype(D))
+    f = s.engine_layout(1).view(fnode_dtype(D))
+    prims = s.engine_layout(4).view(np.float32).reshape(-1, 4)
+    ids = s.engine_layout(5).view(np.int32)
+    big = s.engine_layout(9).view(np.uint32)
+    assert len(q) == len(f) == n_nodes and root == 0
+    n_prims = F.shape[0]
+    assert np.array_equal(np.sort(ids), np.arange(n_prims))
+    pts = prims.reshape(n_prims, 3, 4)[:, :, :3] if D == 3 else prims.reshape(n_prims, 2, 2)
+    np.testing.assert_array_equal(pts, V[F[ids]])
+    covered = np.zeros(n_prims, np.int64)
+    seen = np.zeros(n_nodes, np.int64)
+
+    def subtree_slots(ref):
+        if ref & LEAF:
+            a, c = leaf_range(ref, big)
+            return list(range(a, a + c))
+        out = []
+        nd = f[ref]
+        for j in range(int(nd["nchild"])):
+            out += subtree_slots(int(nd["child"][j]))
+        return out
+
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        seen[i] += 1
+        qn, fn = q[i], f[i]
+        assert qn["nchild"] == fn["nchild"] and 1 <= qn["nchild"] <= 4
+        for j in range(int(fn["nchild"])):
+            ref = int(fn["child"][j])
+            assert ref == int(qn["child"][j])
+            if ref & LEAF:
+                a, c = leaf_range(ref, big)
+                assert c >= 1 and a + c <= n_prims
+                covered[a:a + c] += 1
+            else:
+                assert ref < n_nodes
+                stack.append(ref)
+            slots = subtree_slots(ref)
+            geo = pts[slots].reshape(-1, D)
+            lo = np.array([fn["lo"][k][j] for k in range(D)], np.float32)
+            hi = np.array([fn["hi"][k][j] for k in range(D)], np.float32)
+            assert (geo >= lo).all() and (geo <= hi).all()
+            qlo = decode(qn["qlo"][:, j], qn["exp"], qn["origin"])
+            qhi = decode(qn["qhi"][:, j], qn["exp"], qn["origin"])
+            assert (qlo <= lo).all() and (qhi >= hi).all(), (i, j, qlo, lo, qhi, hi)
+            # the quantized box is tight to one quantization step
+            step = np.ldexp(1.0, qn["exp"].astype(np.int64))
+            assert ((lo - qlo) <= step * 1.0001 + 1e-30).all() and ((qhi - hi) <= step * 1.0001 + 1e-30).all()
+    assert (covered == 1).all() and (seen == 1).all()
+
+
+@pytest.mark.parametrize("name", ["star300", "star8192", "ico1", "ico4"])
+def test_snch_pair_layout(name):
+    V, F = SCENES[name]()
+    D = V.shape[1]
+    s = (ext.SNCH if D == 2 else ext.SNCH3D)(V, F)
+    scal = s.engine_layout(10).view(np.uint32)
+    sroot, n_pairs = scal[1], scal[3]
+    pq = s.engine_layout(2).view(spq_dtype(D))
+    pf = s.engine_layout(3).view(spf_dtype(D))
+    sils = s.engine_layout(6).view(np.float32).reshape(-1, 12 if D == 3 else 8)
+    sil_ids = s.engine_layout(7).view(np.int32)
+    big = s.engine_layout(9).view(np.uint32)
+    ref_sil = s.silhouette_data().reshape(-1, 19 if D == 3 else 10)
+    assert len(pq) == len(pf) == n_pairs and sroot == 0
+    np.testing.assert_array_equal(sil_ids, ref_sil[:, -1].astype(np.int32))
+    if D == 3:
+        np.testing.assert_array_equal(sils[:, 0:3], ref_sil[:, 3:6])   # a = m1
+        np.testing.assert_array_equal(sils[:, 3:6], ref_sil[:, 6:9])   # b = m2
+        seg = np.stack([sils[:, 0:3], sils[:, 3:6]], 1)
+        n0, n1 = sils[:, 6:9], sils[:, 9:12]
+    else:
+        seg = np.stack([sils[:, 0:2], sils[:, 2:4]], 1)
+    covered = np.zeros(len(sils), np.int64)
+
+    def slots(ref):
+        if ref & LEAF:
+            a, c = leaf_range(ref, big)
+            return list(range(a, a + c))
+        return slots(int(pf[ref]["child"][0])) + slots(int(pf[ref]["child"][1]))
+
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        for j in range(2):
+            ref = int(pf[i]["child"][j])
+            assert ref == int(pq[i]["child"][j])
+            if ref & LEAF:
+                a, c = leaf_range(ref, big)
+                assert a + c <= len(sils)
+                covered[a:a + c] += 1
+            else:
+                assert ref < n_pairs
+                stack.append(ref)
+            sl = slots(ref)
+            lo, hi = pf[i]["lo"][j], pf[i]["hi"][j]
+            qlo = decode(pq[i]["qlo"][j], pq[i]["exp"], pq[i]["origin"])
+            qhi = decode(pq[i]["qhi"][j], pq[i]["exp"], pq[i]["origin"])
+            assert (qlo <= lo).all() and (qhi >= hi).all()
+            if D == 3 and sl:
+                geo = seg[sl].reshape(-1, 3)
+                assert (geo >= lo).all() and (geo <= hi).all()
+                cos_t = pf[i]["cos0" if j == 0 else "cos1"]
+                if 0 < cos_t < 1.5:
+                    ax = pf[i]["axis0" if j == 0 else "axis1"].astype(np.float64)
+                    a0 = n0[sl].astype(np.float64) @ ax
+                    a1 = n1[sl].astype(np.float64) @ ax
+                    a0 /= np.linalg.norm(n0[sl].astype(np.float64), axis=1)
+                    a1 /= np.linalg.norm(n1[sl].astype(np.float64), axis=1)
+                    sg = np.sign(a0 + a1)
+                    assert (sg * a0 >= cos_t - 1e-6).all() and (sg * a1 >= cos_t - 1e-6).all()
+                    # the quantized cone is no narrower
+                    qcos = pq[i]["cs"][j][0] / 32767.0
+                    assert qcos <= cos_t + 1e-6
+    assert (covered == 1).all()
