@@ -22,6 +22,10 @@ METHOD_ENGINE = 2
 METHODS = {"plain": METHOD_PLAIN, "engine_fp32": METHOD_ENGINE_FP32, "engine": METHOD_ENGINE}
 
 
+class SceneOptions(C.Structure):
+    _fields_ = [("leaf_prims", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
 class Records(C.Structure):
     _fields_ = [
         ("valid", C.c_void_p),
@@ -53,6 +57,7 @@ SIGNATURES = {
     "gq_build_flat_silhouette_edges": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, _i64p]),
     "gq_scene_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "gq_scene_destroy": (C.c_int, [C.c_void_p]),
+    "gq_scene_create_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "gq_scene_info": (C.c_int, [C.c_void_p, _i64p]),
     "gq_closest_point_bvh": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_intersect_bvh": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),

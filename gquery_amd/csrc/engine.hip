@@ -6,8 +6,8 @@
 // in LDS and whose deeper entries spill to private memory.  Children of a node are visited nearest
 // first.  Pruning is conservative: a node is skipped only when no primitive in it can beat the
 // current answer, so results equal the reference traversal's up to exact ties:
-//  * closest point: box distance^2 against a bound = min(best^2, MinMaxDist^2 of every child box
-//    seen so far), MinMaxDist widened by the quantization step and the builder's padding;
+//  * closest point: squared box distance against best^2 (inflated by 2e-6 relative so float
+//    rounding never drops a box holding an equally near primitive);
 //  * ray: slab test with the entry/exit distances widened by 4 ulp;
 //  * silhouette: the reference's box and normal-cone tests (the cone test is evaluated with the
 //    reference's float32 cone values wherever it can cull) plus a cull of subtrees that provably
@@ -27,10 +27,11 @@ namespace {
 
 using namespace gqd;
 
-constexpr int kBlock = 256;
-constexpr int kLdsDepth = 16;
-constexpr int kSpillDepth = 96;
-constexpr int kStackCap = kLdsDepth + kSpillDepth;
+constexpr int kBlock = 256;     // packet kernels: 4 wavefronts per block
+constexpr int kRayBlock = 64;   // lane-per-query ray kernel: one wavefront per block
+constexpr int kRayStackMax = 96;
+constexpr int kWavePassBlocks = 1024;  // persistent grid of the wave-per-query pass (4 per CU)
+constexpr int kPacketBudget = 3000;    // packet steps per wavefront before its queries are deferred  // lane-stack entries: 96 * 64 lanes * 8 B = 48 KB of LDS per block
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
 
@@ -46,47 +47,77 @@ struct EngineParams {
     float r_max_scalar;
     const uint32_t *perm;
     int n;
+    int stack_depth;          // lane-stack entries (ray kernel)
+    int budget;               // packet steps before a wave hands its queries to the wave-per-query pass (0 = never)
+    uint32_t *defer_count;    // [0] deferred queries, [1] work counter of the wave-per-query pass
+    uint4 *deferred;          // (query index, best distance bits, best slot, 0)
     gq_records out;
 };
 
-// ---- per-lane traversal stack: LDS for the top kLdsDepth entries' slots, private spill below ----
-struct Stack {
-    uint2 *lds;                // this lane's column: slot s at lds[s * kBlock]
-    uint2 spill[kSpillDepth];
-    int sp;
-    __device__ __forceinline__ void push(uint32_t ref, float key) {
-        if (sp >= kStackCap) return;  // unreachable: engine_supported() bounds the depth
-        uint2 e = make_uint2(ref, __float_as_uint(key));
-        if (sp < kLdsDepth) lds[sp * kBlock] = e;
-        else spill[sp - kLdsDepth] = e;
-        ++sp;
-    }
-    // Pops entries until one with key <= limit; returns false when the stack is empty.
-    __device__ __forceinline__ bool pop(float limit, uint32_t &ref) {
-        while (sp > 0) {
-            --sp;
-            uint2 e = sp < kLdsDepth ? lds[sp * kBlock] : spill[sp - kLdsDepth];
-            if (__uint_as_float(e.y) <= limit) {
-                ref = e.x;
-                return true;
-            }
-        }
-        return false;
-    }
-};
+// ---- lane-per-query stack (rays): LDS only, depth chosen per launch from the tree depth, one
+// column per lane of a [depth][blockDim] dynamic shared array.  No private spill area, so the
+// kernels need no scratch memory and their occupancy is set by LDS alone.
+#define GQ_LANE_STACK_DECL                     \
+    extern __shared__ uint2 lane_stack[];      \
+    const int lane_col = threadIdx.x;          \
+    const int lane_stride = blockDim.x;        \
+    int sp = 0;
+#define GQ_LPUSH(REF, KEY)                                                                       \
+    do {                                                                                         \
+        if (sp < P.stack_depth) {                                                                \
+            lane_stack[sp * lane_stride + lane_col] = make_uint2((REF), __float_as_uint(KEY));  \
+            ++sp;                                                                                \
+        }                                                                                        \
+    } while (0)
+#define GQ_LPOP(LIMIT, CUR, GO)                                                                  \
+    do {                                                                                         \
+        GO = false;                                                                              \
+        while (sp > 0) {                                                                         \
+            --sp;                                                                                \
+            const uint2 e_ = lane_stack[sp * lane_stride + lane_col];                            \
+            if (__uint_as_float(e_.y) <= (LIMIT)) {                                              \
+                CUR = e_.x;                                                                      \
+                GO = true;                                                                       \
+                break;                                                                           \
+            }                                                                                    \
+        }                                                                                        \
+    } while (0)
 
-template <typename T> __device__ __forceinline__ T load_rec(const T *p) {
-    static_assert(sizeof(T) % 16 == 0, "records are multiples of 16 bytes");
-    union U {
-        T t;
-        uint4 v[sizeof(T) / 16];
-        __device__ U() {}
-    } u;
-    const uint4 *s = reinterpret_cast<const uint4 *>(p);
+// ---- wave-coherent ("packet") traversal helpers: the 64 lanes of a wavefront hold 64 nearby
+// queries (Morton order) and walk the tree together; the stack holds node references only and
+// is shared by the wave, so it lives in a few LDS words; node and primitive records are read at
+// wave-uniform addresses (scalar loads).
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniformf(float v) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v))); }
+__device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
-    for (int k = 0; k < (int)(sizeof(T) / 16); ++k) u.v[k] = s[k];
-    return u.t;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
+
+__device__ __forceinline__ uint4 ld16(const void *p, int k) { return reinterpret_cast<const uint4 *>(p)[k]; }
+__device__ __forceinline__ float ubyte(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xFFu); }
+__device__ __forceinline__ float ushort(uint32_t w, int j) { return (float)((w >> (16 * j)) & 0xFFFFu); }
+__device__ __forceinline__ int sbyte(uint32_t w, int j) { return (int)(int8_t)((w >> (8 * j)) & 0xFFu); }
+__device__ __forceinline__ uint32_t comp4(const uint4 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// A record as 32-bit words, loaded with 16-byte loads; fields are read at constant byte offsets.
+template <int NW> struct Words {
+    uint32_t w[NW];
+    __device__ __forceinline__ void load(const void *p) {
+#pragma unroll
+        for (int k = 0; k < NW / 4; ++k) {
+            const uint4 v = ld16(p, k);
+            w[4 * k] = v.x; w[4 * k + 1] = v.y; w[4 * k + 2] = v.z; w[4 * k + 3] = v.w;
+        }
+    }
+    __device__ __forceinline__ uint32_t u32(int off) const { return w[off / 4]; }
+    __device__ __forceinline__ float f32(int off) const { return __uint_as_float(w[off / 4]); }
+    __device__ __forceinline__ uint32_t u8(int off) const { return (w[off / 4] >> (8 * (off % 4))) & 0xFFu; }
+    __device__ __forceinline__ int s8(int off) const { return (int)(int8_t)u8(off); }
+    __device__ __forceinline__ uint32_t u16(int off) const { return (w[off / 4] >> (8 * (off % 4))) & 0xFFFFu; }
+    __device__ __forceinline__ int s16(int off) const { return (int)(int16_t)u16(off); }
+};
 
 __device__ __forceinline__ void leaf_range(uint32_t ref, const uint32_t *big, uint32_t &first, uint32_t &count) {
     first = ref_first(ref);
@@ -103,42 +134,66 @@ __device__ __forceinline__ float pad_of(float c) { return 2.5e-7f * (1.0f + fabs
 // Child boxes of a BVH4 node, decoded; delta[k] bounds how far a child face may lie outside the
 // nearest primitive point (quantization step + builder padding).
 template <int D, bool Q> struct Node4 {
-    float lo[D][4], hi[D][4], delta[D];
+    float lo[D][4], hi[D][4], step[D];   // step: quantization step per axis (0 for float32 boxes)
     uint32_t child[4];
     int n;
 };
 
 template <int D, bool Q>
 __device__ __forceinline__ void load_node4(const void *base, uint32_t idx, Node4<D, Q> &nd) {
-    if constexpr (Q) {
-        const QNode<D> q = load_rec(reinterpret_cast<const QNode<D> *>(base) + idx);
-        nd.n = q.nchild;
+    if constexpr (Q && D == 3) {
+        // QNode<3>: [origin xyz | exp xyz, nchild] [qlo x,y,z | qhi x] [qhi y,z | child 0,1] [child 2,3 | pad]
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64;
+        const uint4 w0 = ld16(p, 0), w1 = ld16(p, 1), w2 = ld16(p, 2), w3 = ld16(p, 3);
+        const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+        nd.n = (int)(w0.w >> 24);
+        const uint32_t qlo[3] = {w1.x, w1.y, w1.z}, qhi[3] = {w1.w, w2.x, w2.y};
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const float s = ldexpf(1.0f, (int)q.exp[k]);
-            nd.delta[k] = s + pad_of(q.origin[k]) + pad_of(fmaf((float)QBits<D>::levels, s, q.origin[k]));
+        for (int k = 0; k < 3; ++k) {
+            const float sc = ldexpf(1.0f, sbyte(w0.w, k));
+            nd.step[k] = sc;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                nd.lo[k][j] = fmaf((float)q.qlo[k][j], s, q.origin[k]);
-                nd.hi[k][j] = fmaf((float)q.qhi[k][j], s, q.origin[k]);
+                nd.lo[k][j] = fmaf(ubyte(qlo[k], j), sc, org[k]);
+                nd.hi[k][j] = fmaf(ubyte(qhi[k], j), sc, org[k]);
             }
         }
+        nd.child[0] = w2.z; nd.child[1] = w2.w; nd.child[2] = w3.x; nd.child[3] = w3.y;
+    } else if constexpr (Q && D == 2) {
+        // QNode<2>: [origin xy | exp xy, nchild, pad | qlo x0 x1] [qlo x2 x3, y0..y3 | qhi x0 x1] ...
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64;
+        const uint4 w0 = ld16(p, 0), w1 = ld16(p, 1), w2 = ld16(p, 2), w3 = ld16(p, 3);
+        const float org[2] = {__uint_as_float(w0.x), __uint_as_float(w0.y)};
+        nd.n = (int)((w0.z >> 16) & 0xFFu);
+        // 16-bit fields from byte 12: qlo[2][4] then qhi[2][4] (32 bytes, 8 words)
+        const uint32_t q[8] = {w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) nd.child[j] = q.child[j];
+        for (int k = 0; k < 2; ++k) {
+            const float sc = ldexpf(1.0f, sbyte(w0.z, k));
+            nd.step[k] = sc;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                nd.lo[k][j] = fmaf(ushort(q[k * 2 + j / 2], j & 1), sc, org[k]);
+                nd.hi[k][j] = fmaf(ushort(q[4 + k * 2 + j / 2], j & 1), sc, org[k]);
+            }
+        }
+        nd.child[0] = w2.w; nd.child[1] = w3.x; nd.child[2] = w3.y; nd.child[3] = w3.z;
     } else {
-        const FNode<D> f = load_rec(reinterpret_cast<const FNode<D> *>(base) + idx);
-        nd.n = (int)f.nchild;
+        // FNode<D>: lo[D][4], hi[D][4] as 2D float4, then child[4], nchild
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 128;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            nd.delta[k] = 0.0f;
+            const uint4 l = ld16(p, k), h = ld16(p, D + k);
+            nd.step[k] = 0.0f;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                nd.lo[k][j] = f.lo[k][j];
-                nd.hi[k][j] = f.hi[k][j];
+                nd.lo[k][j] = __uint_as_float(comp4(l, j));
+                nd.hi[k][j] = __uint_as_float(comp4(h, j));
             }
         }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) nd.child[j] = f.child[j];
+        const uint4 c = ld16(p, 2 * D), m = ld16(p, 2 * D + 1);
+        nd.child[0] = c.x; nd.child[1] = c.y; nd.child[2] = c.z; nd.child[3] = c.w;
+        nd.n = (int)m.x;
     }
 }
 
@@ -168,91 +223,10 @@ template <int D> __device__ __forceinline__ uint32_t query_index(const EnginePar
     return P.perm ? P.perm[i] : (uint32_t)i;
 }
 
-// ---------------------------------------------------------------------------------------------
-// closest point
-template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
+// Output record of a closest-point query from the winning primitive slot (reference formulas).
+template <int D>
+__device__ __forceinline__ void cp_record(const EngineParams &P, uint32_t qi, const typename VT<D>::T &p, int best_slot) {
     using V = typename VT<D>::T;
-    __shared__ uint2 lds[kLdsDepth * kBlock];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.n) return;
-    const uint32_t qi = query_index<D>(P, i);
-    const V p = ldv<D>(P.x + (size_t)D * qi);
-    float pc[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) pc[k] = comp<D>(p, k);
-    Stack stk;
-    stk.lds = lds + threadIdx.x;
-    stk.sp = 0;
-    float best = kInf, bound2 = kInf;
-    int best_slot = -1;
-    uint32_t cur = P.root;
-    bool go = true;
-    while (go) {
-        if (!ref_is_leaf(cur)) {
-            Node4<D, Q> nd;
-            load_node4<D, Q>(P.nodes, cur, nd);
-            float key[4];
-            uint32_t ref[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                float d2 = 0.0f, fsum = 0.0f, far2[D], near2[D];
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float a = nd.lo[k][j] - pc[k], b = pc[k] - nd.hi[k][j];
-                    const float o = fmaxf(fmaxf(a, b), 0.0f);
-                    d2 = fmaf(o, o, d2);
-                    const float fa = fabsf(a), fb = fabsf(b);
-                    const float nr = fminf(fa, fb) + nd.delta[k];
-                    const float fr = fmaxf(fa, fb);
-                    far2[k] = fr * fr;
-                    near2[k] = nr * nr;
-                    fsum += far2[k];
-                }
-                float ub2 = kInf;
-#pragma unroll
-                for (int k = 0; k < D; ++k) ub2 = fminf(ub2, fsum - far2[k] + near2[k]);
-                const bool valid = j < nd.n;
-                if (valid) bound2 = fminf(bound2, ub2 * 1.000002f);
-                key[j] = valid ? d2 : kInf;
-                ref[j] = nd.child[j];
-            }
-            sort4(key, ref);
-            int nh = 0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) nh += (key[j] <= bound2) & (key[j] < kInf);
-            if (nh == 0) {
-                go = stk.pop(bound2, cur);
-                continue;
-            }
-#pragma unroll
-            for (int j = 3; j >= 1; --j)
-                if (j < nh) stk.push(ref[j], key[j]);
-            cur = ref[0];
-        } else {
-            uint32_t first, count;
-            leaf_range(cur, P.big, first, count);
-            for (uint32_t s = first; s < first + count; ++s) {
-                float dd;
-                if constexpr (D == 3) {
-                    const float4 a = P.prims[3 * s], b = P.prims[3 * s + 1], c = P.prims[3 * s + 2];
-                    V3 pt;
-                    V2 uv;
-                    dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
-                } else {
-                    const float4 ab = P.prims[s];
-                    V2 pt;
-                    dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
-                }
-                if (dd < best) {
-                    best = dd;
-                    best_slot = (int)s;
-                    bound2 = fminf(bound2, dd * dd * 1.000002f);
-                }
-            }
-            go = stk.pop(bound2, cur);
-        }
-    }
     const gq_records &o = P.out;
     if (best_slot < 0) {
         if (o.valid) o.valid[qi] = 0;
@@ -266,14 +240,14 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
     const int s = best_slot;
     if constexpr (D == 3) {
         // Triangle.closest_point (triangle.py:75-81)
-        const float4 a4 = P.prims[3 * s], b4 = P.prims[3 * s + 1], c4 = P.prims[3 * s + 2];
+        const float4 a4 = P.prims[4 * s], b4 = P.prims[4 * s + 1], c4 = P.prims[4 * s + 2];
         const V3 a = v3(a4.x, a4.y, a4.z), b = v3(b4.x, b4.y, b4.z), c = v3(c4.x, c4.y, c4.z);
         V3 pt;
         V2 uv;
         const float dd = closest_point_triangle(p, a, b, c, pt, uv);
-        if (o.p) st(o.p + 3 * (size_t)qi, pt);
-        if (o.n) st(o.n + 3 * (size_t)qi, normalize(cross(b - a, c - a)));
-        if (o.uv) st(o.uv + 2 * (size_t)qi, uv);
+        if (o.p) gqd::st(o.p + 3 * (size_t)qi, pt);
+        if (o.n) gqd::st(o.n + 3 * (size_t)qi, normalize(cross(b - a, c - a)));
+        if (o.uv) gqd::st(o.uv + 2 * (size_t)qi, uv);
         if (o.d) o.d[qi] = dd;
     } else {
         // LineSegment.closest_point (line_segment.py:63-78)
@@ -283,8 +257,8 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
         const V2 t = normalize(ba);
         const float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
         const V2 ip = a + ba * h;
-        if (o.p) st(o.p + 2 * (size_t)qi, ip);
-        if (o.n) st(o.n + 2 * (size_t)qi, v2(t.y, -t.x));
+        if (o.p) gqd::st(o.p + 2 * (size_t)qi, ip);
+        if (o.n) gqd::st(o.n + 2 * (size_t)qi, v2(t.y, -t.x));
         if (o.uv) o.uv[qi] = h;
         if (o.d) o.d[qi] = norm(ip - p);
     }
@@ -293,12 +267,250 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// ray
+// closest point (packet)
+constexpr int kPacketDepth = 128;   // wave stack entries (node references); bounded by engine_supported()
+
+template <int D> __device__ __forceinline__ void packet_centroid(bool live, const float *pc, float *cen) {
+    const float nl = fmaxf(wave_sum(live ? 1.0f : 0.0f), 1.0f);
+#pragma unroll
+    for (int k = 0; k < D; ++k) cen[k] = uniformf(wave_sum(live ? pc[k] : 0.0f) / nl);
+}
+
 template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
+__global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
     using V = typename VT<D>::T;
-    __shared__ uint2 lds[kLdsDepth * kBlock];
+    __shared__ uint32_t wstack[kBlock / 64][kPacketDepth];
+    uint32_t *wst = wstack[threadIdx.x >> 6];
     const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < P.n;
+    const uint32_t qi = live ? query_index<D>(P, i) : 0u;
+    V p;
+    if (live) p = ldv<D>(P.x + (size_t)D * qi);
+    else p = ldv<D>(P.x);   // any valid point; the lane's result is discarded
+    float pc[D], cen[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) pc[k] = comp<D>(p, k);
+    packet_centroid<D>(live, pc, cen);
+    float best = kInf, bound2 = live ? kInf : -1.0f;   // a dead lane never opens anything
+    int best_slot = -1;
+    int sp = 0;
+    int steps = 0;
+    uint32_t cur = uniform(P.root);
+    for (;;) {
+        if (P.budget > 0 && ++steps > P.budget) {
+            // Too much work for one wavefront (queries deep inside a closed surface see most of it
+            // at nearly one distance): hand the queries, with the answer so far as the starting
+            // bound, to the wave-per-query pass so they spread over many wavefronts.
+            const unsigned long long m = __ballot(live);
+            const uint32_t nlive = (uint32_t)__popcll(m);
+            uint32_t base = 0;
+            if ((threadIdx.x & 63) == 0) base = atomicAdd(P.defer_count, nlive);
+            base = uniform(__shfl(base, 0, 64));
+            if (live) {
+                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
+                P.deferred[base + rank] = make_uint4(qi, __float_as_uint(best), (uint32_t)best_slot, 0u);
+            }
+            return;
+        }
+        if (!ref_is_leaf(cur)) {
+            Node4<D, Q> nd;
+            load_node4<D, Q>(P.nodes, cur, nd);
+            float key[4];
+            uint32_t ref[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float d2 = 0.0f, c2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float o = fmaxf(fmaxf(nd.lo[k][j] - pc[k], pc[k] - nd.hi[k][j]), 0.0f);
+                    d2 = fmaf(o, o, d2);
+                    const float oc = fmaxf(fmaxf(nd.lo[k][j] - cen[k], cen[k] - nd.hi[k][j]), 0.0f);
+                    c2 = fmaf(oc, oc, c2);
+                }
+                const bool any = __ballot((j < nd.n) && (d2 <= bound2)) != 0ull;
+                key[j] = any ? uniformf(c2) : kInf;
+                ref[j] = nd.child[j];
+            }
+            sort4(key, ref);
+            int nh = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) nh += key[j] < kInf;
+            if (nh > 0) {
+#pragma unroll
+                for (int j = 3; j >= 1; --j)
+                    if (j < nh && sp < kPacketDepth) wst[sp++] = ref[j];
+                cur = uniform(ref[0]);
+                continue;
+            }
+        } else {
+            uint32_t first, count;
+            leaf_range(cur, P.big, first, count);
+            first = uniform(first);
+            count = uniform(count);
+            for (uint32_t s = first; s < first + count; ++s) {
+                float dd;
+                if constexpr (D == 3) {
+                    const float4 a = P.prims[4 * s], b = P.prims[4 * s + 1], c = P.prims[4 * s + 2];
+                    const float4 pl = P.prims[4 * s + 3];
+                    // plane-distance lower bound, widened for rounding: skip the exact test when no
+                    // lane can get strictly closer through this triangle
+                    const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
+                    if (__ballot(lb * 0.99999f - 1e-6f < best) == 0ull) continue;
+                    V3 pt;
+                    V2 uv;
+                    dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
+                } else {
+                    const float4 ab = P.prims[s];
+                    V2 pt;
+                    dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
+                }
+                if (live && dd < best) {
+                    best = dd;
+                    best_slot = (int)s;
+                    bound2 = dd * dd * 1.000002f;
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = uniform(wst[--sp]);
+    }
+    if (!live) return;
+    cp_record<D>(P, qi, p, best_slot);
+}
+
+// ---------------------------------------------------------------------------------------------
+// closest point, wave per query: finishes the queries the packet pass deferred.  The 64 lanes
+// expand the top 64 frontier entries per round and append the opened children in lane order, so
+// the frontier stays sorted by depth; every append at depth d happens after all older depth-d
+// entries were taken, hence at most 256 entries per depth level and 256 * (levels) in total.
+// Leaf candidates are reduced across the wave each round.
+constexpr int kWaveBlock = 128;
+constexpr int kFront = 2560;
+
+__device__ __forceinline__ int wave_exclusive_scan(int v, int &total) {
+    const int lane = threadIdx.x & 63;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+template <int D, bool Q>
+__global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams P) {
+    using V = typename VT<D>::T;
+    __shared__ uint2 front_all[kWaveBlock / 64][kFront];
+    uint2 *front = front_all[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const uint32_t total = uniform(P.defer_count[0]);
+    for (;;) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(P.defer_count + 1, 1u);
+        k = uniform(__shfl(k, 0, 64));
+        if (k >= total) break;
+        const uint4 item = P.deferred[k];
+        const uint32_t qi = uniform(item.x);
+        float best = uniformf(__uint_as_float(item.y));
+        int best_slot = (int)uniform(item.z);
+        const V p = ldv<D>(P.x + (size_t)D * qi);
+        float pc[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) pc[j] = comp<D>(p, j);
+        float bound2 = best < kInf ? best * best * 1.000002f : kInf;
+        int size = 1;
+        front[0] = make_uint2(P.root, 0u);
+        while (size > 0) {
+            const int take = size < 64 ? size : 64;
+            uint2 e = make_uint2(0u, __float_as_uint(kInf));
+            if (lane < take) e = front[size - take + lane];
+            size -= take;
+            uint32_t child[4];
+            float ckey[4];
+            int nh = 0;
+            float cand = kInf;
+            int cand_slot = -1;
+            if (lane < take && __uint_as_float(e.y) <= bound2) {
+                if (!ref_is_leaf(e.x)) {
+                    Node4<D, Q> nd;
+                    load_node4<D, Q>(P.nodes, e.x, nd);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        float d2 = 0.0f;
+#pragma unroll
+                        for (int c = 0; c < D; ++c) {
+                            const float o = fmaxf(fmaxf(nd.lo[c][j] - pc[c], pc[c] - nd.hi[c][j]), 0.0f);
+                            d2 = fmaf(o, o, d2);
+                        }
+                        if (j < nd.n && d2 <= bound2) {
+                            child[nh] = nd.child[j];
+                            ckey[nh] = d2;
+                            ++nh;
+                        }
+                    }
+                } else {
+                    uint32_t first, count;
+                    leaf_range(e.x, P.big, first, count);
+                    for (uint32_t s = first; s < first + count; ++s) {
+                        float dd;
+                        if constexpr (D == 3) {
+                            const float4 pl = P.prims[4 * s + 3];
+                            const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
+                            if (lb * 0.99999f - 1e-6f >= best) continue;
+                            const float4 a = P.prims[4 * s], b = P.prims[4 * s + 1], c = P.prims[4 * s + 2];
+                            V3 pt;
+                            V2 uv;
+                            dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
+                        } else {
+                            const float4 ab = P.prims[s];
+                            V2 pt;
+                            dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
+                        }
+                        if (dd < cand || (dd == cand && (int)s < cand_slot)) {
+                            cand = dd;
+                            cand_slot = (int)s;
+                        }
+                    }
+                }
+            }
+            // wave minimum of the leaf candidates (ties to the lower slot, so every lane agrees)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const float oc = __shfl_xor(cand, o, 64);
+                const int os = __shfl_xor(cand_slot, o, 64);
+                if (oc < cand || (oc == cand && os < cand_slot)) {
+                    cand = oc;
+                    cand_slot = os;
+                }
+            }
+            cand = uniformf(cand);
+            cand_slot = (int)uniform((uint32_t)cand_slot);
+            if (cand < best) {
+                best = cand;
+                best_slot = cand_slot;
+                bound2 = cand * cand * 1.000002f;
+            }
+            // append the opened children
+            int tot;
+            const int off = wave_exclusive_scan(nh, tot);
+            tot = (int)uniform((uint32_t)tot);
+            for (int j = 0; j < nh; ++j)
+                if (size + off + j < kFront) front[size + off + j] = make_uint2(child[j], __float_as_uint(ckey[j]));
+            size = size + tot < kFront ? size + tot : kFront;
+        }
+        if (lane == 0) cp_record<D>(P, qi, p, best_slot);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// ray (lane per query)
+template <int D, bool Q>
+__global__ void __launch_bounds__(kRayBlock) k_intersect(EngineParams P) {
+    using V = typename VT<D>::T;
+    GQ_LANE_STACK_DECL
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t qi = query_index<D>(P, i);
     const V x = ldv<D>(P.x + (size_t)D * qi);
@@ -310,10 +522,6 @@ __global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
         inv[k] = 1.0f / comp<D>(v, k);
     }
     float best = P.r_max ? P.r_max[qi] : P.r_max_scalar;
-    const float r_max0 = best;
-    Stack stk;
-    stk.lds = lds + threadIdx.x;
-    stk.sp = 0;
     int best_slot = -1;
     uint32_t cur = P.root;
     bool go = true;
@@ -345,12 +553,12 @@ __global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) nh += (key[j] <= lim) & (key[j] < kInf);
             if (nh == 0) {
-                go = stk.pop(best * kHi, cur);
+                GQ_LPOP(best * kHi, cur, go);
                 continue;
             }
 #pragma unroll
             for (int j = 3; j >= 1; --j)
-                if (j < nh) stk.push(ref[j], key[j]);
+                if (j < nh) GQ_LPUSH(ref[j], key[j]);
             cur = ref[0];
         } else {
             uint32_t first, count;
@@ -358,7 +566,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
             for (uint32_t s = first; s < first + count; ++s) {
                 float t;
                 if constexpr (D == 3) {
-                    const float4 a = P.prims[3 * s], b = P.prims[3 * s + 1], c = P.prims[3 * s + 2];
+                    const float4 a = P.prims[4 * s], b = P.prims[4 * s + 1], c = P.prims[4 * s + 2];
                     t = ray_triangle_t(x, v, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z));
                 } else {
                     const float4 ab = P.prims[s];
@@ -369,7 +577,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
                     best_slot = (int)s;
                 }
             }
-            go = stk.pop(best * kHi, cur);
+            GQ_LPOP(best * kHi, cur, go);
         }
     }
     const gq_records &o = P.out;
@@ -378,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
     if (ok) {
         if constexpr (D == 3) {
             // Triangle.ray_intersect (triangle.py:48-73) with r_max = the hit distance
-            const float4 a4 = P.prims[3 * s], b4 = P.prims[3 * s + 1], c4 = P.prims[3 * s + 2];
+            const float4 a4 = P.prims[4 * s], b4 = P.prims[4 * s + 1], c4 = P.prims[4 * s + 2];
             const V3 a = v3(a4.x, a4.y, a4.z), b = v3(b4.x, b4.y, b4.z), c = v3(c4.x, c4.y, c4.z);
             const V3 v1 = b - a, v2_ = c - a;
             const V3 pp = cross(v, v2_);
@@ -399,22 +607,21 @@ __global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
                 }
             }
             if (ok) {
-                if (o.p) st(o.p + 3 * (size_t)qi, a + v1 * vv + v2_ * ww);
-                if (o.n) st(o.n + 3 * (size_t)qi, normalize(cross(v1, v2_)));
-                if (o.uv) st(o.uv + 2 * (size_t)qi, v2(1.0f - vv - ww, vv));
+                if (o.p) gqd::st(o.p + 3 * (size_t)qi, a + v1 * vv + v2_ * ww);
+                if (o.n) gqd::st(o.n + 3 * (size_t)qi, normalize(cross(v1, v2_)));
+                if (o.uv) gqd::st(o.uv + 2 * (size_t)qi, v2(1.0f - vv - ww, vv));
                 if (o.d) o.d[qi] = t;
             }
         } else {
             // record of bvh.py:267-274
             const float4 ab = P.prims[s];
             const V2 tt = v2(ab.z - ab.x, ab.w - ab.y);
-            if (o.p) st(o.p + 2 * (size_t)qi, x + v * best);
-            if (o.n) st(o.n + 2 * (size_t)qi, normalize(v2(tt.y, -tt.x)));
+            if (o.p) gqd::st(o.p + 2 * (size_t)qi, x + v * best);
+            if (o.n) gqd::st(o.n + 2 * (size_t)qi, normalize(v2(tt.y, -tt.x)));
             if (o.uv) o.uv[qi] = -1.0f;
             if (o.d) o.d[qi] = best;
         }
     }
-    (void)r_max0;
     if (!ok) {
         if (o.valid) o.valid[qi] = 0;
         if (o.p) for (int k = 0; k < D; ++k) o.p[(size_t)D * qi + k] = 0.0f;
@@ -431,7 +638,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(EngineParams P) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// silhouette
+// silhouette (packet)
 struct PairChild {
     float lo[3], hi[3];
     float axis[3], cos_t, sin_t;
@@ -442,45 +649,59 @@ struct PairChild {
 template <int D, bool Q>
 __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairChild (&c)[2]) {
     if constexpr (Q) {
-        const SPairQ<D> q = load_rec(reinterpret_cast<const SPairQ<D> *>(base) + idx);
+        // SPairQ<D> (64 B): origin, exp | child boxes | cone_axis int16 @28 | cone_cs int16 @40 |
+        // child @48 | ref_cone @56
+        Words<16> r;
+        r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const float s = ldexpf(1.0f, (int)q.exp[k]);
+            const float org = r.f32(4 * k);
+            const float sc = ldexpf(1.0f, r.s8(4 * D + k));
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                c[j].lo[k] = fmaf((float)q.qlo[j][k], s, q.origin[k]);
-                c[j].hi[k] = fmaf((float)q.qhi[j][k], s, q.origin[k]);
+                if constexpr (D == 3) {
+                    c[j].lo[k] = fmaf((float)r.u8(16 + 3 * j + k), sc, org);
+                    c[j].hi[k] = fmaf((float)r.u8(22 + 3 * j + k), sc, org);
+                } else {
+                    c[j].lo[k] = fmaf((float)r.u16(12 + 2 * (2 * j + k)), sc, org);
+                    c[j].hi[k] = fmaf((float)r.u16(20 + 2 * (2 * j + k)), sc, org);
+                }
             }
         }
+        constexpr float kS = 1.0f / 32767.0f;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            constexpr float kS = 1.0f / 32767.0f;
-            float a0 = q.cone_axis[j][0] * kS, a1 = q.cone_axis[j][1] * kS, a2 = q.cone_axis[j][2] * kS;
-            float l = a0 * a0 + a1 * a1 + a2 * a2;
-            float r = l > 0.0f ? 1.0f / sqrtf(l) : 0.0f;
-            c[j].axis[0] = a0 * r;
-            c[j].axis[1] = a1 * r;
-            c[j].axis[2] = a2 * r;
-            c[j].cos_t = q.cone_cs[j][0] == 32767 && q.cone_cs[j][1] == -32767 ? 2.0f : q.cone_cs[j][0] * kS;
-            c[j].sin_t = q.cone_cs[j][1] * kS;
-            c[j].child = q.child[j];
-            c[j].ref_cone = q.ref_cone[j];
+            const float a0 = r.s16(28 + 6 * j) * kS, a1 = r.s16(30 + 6 * j) * kS, a2 = r.s16(32 + 6 * j) * kS;
+            const float l = a0 * a0 + a1 * a1 + a2 * a2;
+            const float rn = l > 0.0f ? 1.0f / sqrtf(l) : 0.0f;
+            c[j].axis[0] = a0 * rn;
+            c[j].axis[1] = a1 * rn;
+            c[j].axis[2] = a2 * rn;
+            const int cs0 = r.s16(40 + 4 * j), cs1 = r.s16(42 + 4 * j);
+            c[j].cos_t = (cs0 == 32767 && cs1 == -32767) ? 2.0f : cs0 * kS;
+            c[j].sin_t = cs1 * kS;
+            c[j].child = r.u32(48 + 4 * j);
+            c[j].ref_cone = (int32_t)r.u32(56 + 4 * j);
         }
     } else {
-        const SPairF<D> f = load_rec(reinterpret_cast<const SPairF<D> *>(base) + idx);
+        // SPairF<D> (128 B): lo[2][D] @0, hi[2][D] @8D, cones (axis3, cos, sin) @16D, child @16D+40,
+        // ref_cone @16D+48
+        Words<32> r;
+        r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 128);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                c[j].lo[k] = f.lo[j][k];
-                c[j].hi[k] = f.hi[j][k];
+                c[j].lo[k] = r.f32(4 * (j * D + k));
+                c[j].hi[k] = r.f32(8 * D + 4 * (j * D + k));
             }
+            const int co = 16 * D + 20 * j;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) c[j].axis[k] = f.cone[j].axis[k];
-            c[j].cos_t = f.cone[j].cos_t;
-            c[j].sin_t = f.cone[j].sin_t;
-            c[j].child = f.child[j];
-            c[j].ref_cone = f.ref_cone[j];
+            for (int k = 0; k < 3; ++k) c[j].axis[k] = r.f32(co + 4 * k);
+            c[j].cos_t = r.f32(co + 12);
+            c[j].sin_t = r.f32(co + 16);
+            c[j].child = r.u32(16 * D + 40 + 4 * j);
+            c[j].ref_cone = (int32_t)r.u32(16 * D + 48 + 4 * j);
         }
     }
 }
@@ -512,36 +733,40 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
     }
 }
 
+
 template <int D, bool Q>
 __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
     using V = typename VT<D>::T;
-    __shared__ uint2 lds[kLdsDepth * kBlock];
+    __shared__ uint32_t wstack[kBlock / 64][kPacketDepth];
+    uint32_t *wst = wstack[threadIdx.x >> 6];
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.n) return;
-    const uint32_t qi = query_index<D>(P, i);
-    const V x = ldv<D>(P.x + (size_t)D * qi);
-    float xc[3] = {0.0f, 0.0f, 0.0f};
+    const bool live = i < P.n;
+    const uint32_t qi = live ? query_index<D>(P, i) : 0u;
+    V x;
+    if (live) x = ldv<D>(P.x + (size_t)D * qi);
+    else x = ldv<D>(P.x);
+    float xc[3] = {0.0f, 0.0f, 0.0f}, cen[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = comp<D>(x, k);
-    float r = P.r_max ? P.r_max[qi] : P.r_max_scalar;
+    packet_centroid<D>(live, xc, cen);
+    float r = live ? (P.r_max ? P.r_max[qi] : P.r_max_scalar) : -1.0f;
     bool found = false;
     V best_p{};
     float best_d = 0.0f;
     int best_slot = 0;
     float d_root, d_unused;
     box_distance(ldv<D>(P.root_lo), ldv<D>(P.root_hi), x, d_root, d_unused);
-    if (d_root <= r) {
-        Stack stk;
-        stk.lds = lds + threadIdx.x;
-        stk.sp = 0;
-        uint32_t cur = P.root;
-        bool go = true;
-        while (go) {
+    // a lane whose query misses the root box takes no part (reference: snch.py:111-112)
+    if (!(d_root <= r)) r = -1.0f;
+    if (__ballot(r >= 0.0f) != 0ull) {
+        int sp = 0;
+        uint32_t cur = uniform(P.root);
+        for (;;) {
             if (!ref_is_leaf(cur)) {
                 PairChild c[2];
                 load_pair<D, Q>(P.nodes, cur, c);
+                bool any[2];
                 float key[2];
-                bool hit[2];
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     float dm, dx;
@@ -552,21 +777,30 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                         h = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dm);
                     }
                     if (h && exact_cull<D>(c[j], xc)) h = false;
-                    hit[j] = h;
-                    key[j] = dm;
+                    any[j] = __ballot(h) != 0ull;
+                    float c2 = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        const float oc = fmaxf(fmaxf(c[j].lo[k] - cen[k], cen[k] - c[j].hi[k]), 0.0f);
+                        c2 = fmaf(oc, oc, c2);
+                    }
+                    key[j] = uniformf(c2);
                 }
-                if (hit[0] & hit[1]) {
+                if (any[0] && any[1]) {
                     const int near = key[1] < key[0] ? 1 : 0;
-                    stk.push(c[1 - near].child, key[1 - near]);
-                    cur = c[near].child;
-                } else if (hit[0] | hit[1]) {
-                    cur = hit[0] ? c[0].child : c[1].child;
-                } else {
-                    go = stk.pop(r, cur);
+                    if (sp < kPacketDepth) wst[sp++] = c[1 - near].child;
+                    cur = uniform(c[near].child);
+                    continue;
+                }
+                if (any[0] || any[1]) {
+                    cur = uniform(any[0] ? c[0].child : c[1].child);
+                    continue;
                 }
             } else {
                 uint32_t first, count;
                 leaf_range(cur, P.big, first, count);
+                first = uniform(first);
+                count = uniform(count);
                 for (uint32_t s = first; s < first + count; ++s) {
                     if constexpr (D == 3) {
                         const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1], s2 = P.prims[3 * s + 2];
@@ -597,10 +831,12 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                         }
                     }
                 }
-                go = stk.pop(r, cur);
             }
+            if (sp == 0) break;
+            cur = uniform(wst[--sp]);
         }
     }
+    if (!live) return;
     const gq_records &o = P.out;
     if (!found) {
         if (o.valid) o.valid[qi] = 0;
@@ -610,7 +846,7 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
         return;
     }
     if (o.valid) o.valid[qi] = 1;
-    if (o.p) st(o.p + (size_t)D * qi, best_p);
+    if (o.p) gqd::st(o.p + (size_t)D * qi, best_p);
     if (o.d) o.d[qi] = best_d;
     if (o.prim_id) o.prim_id[qi] = P.ids[best_slot];
 }
@@ -623,6 +859,10 @@ __device__ __forceinline__ uint32_t f2ord(float f) {
 }
 __device__ __forceinline__ float ord2f(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void k_zero_u32(uint32_t *b, int n) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) b[t] = 0u;
 }
 
 __global__ void k_bounds_init(uint32_t *b) {
@@ -704,14 +944,25 @@ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 template <int D, bool Q>
 hipError_t launch_traversal(QueryKind kind, const EngineParams &P, hipStream_t st) {
-    const unsigned grid = (unsigned)((P.n + kBlock - 1) / kBlock);
     static const char *names[2][3] = {
         {"engine_closest_point_2d", "engine_intersect_2d", "engine_silhouette_2d"},
         {"engine_closest_point_3d", "engine_intersect_3d", "engine_silhouette_3d"}};
     prof_begin(st);
-    if (kind == kClosestPoint) hipLaunchKernelGGL((k_closest_point<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
-    else if (kind == kIntersect) hipLaunchKernelGGL((k_intersect<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
-    else hipLaunchKernelGGL((k_silhouette<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
+    if (kind == kIntersect) {
+        const unsigned grid = (unsigned)((P.n + kRayBlock - 1) / kRayBlock);
+        const size_t lds = (size_t)P.stack_depth * kRayBlock * sizeof(uint2);
+        hipLaunchKernelGGL((k_intersect<D, Q>), dim3(grid), dim3(kRayBlock), lds, st, P);
+    } else {
+        const unsigned grid = (unsigned)((P.n + kBlock - 1) / kBlock);
+        if (kind == kClosestPoint) {
+            if (P.budget > 0) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, P.defer_count, 2);
+            hipLaunchKernelGGL((k_closest_point<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
+            if (P.budget > 0)
+                hipLaunchKernelGGL((k_closest_point_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, P);
+        } else {
+            hipLaunchKernelGGL((k_silhouette<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
+        }
+    }
     prof_end(st, names[D - 2][kind]);
     return hipGetLastError();
 }
@@ -759,6 +1010,7 @@ hipError_t launch_dim(QueryKind kind, const WideView &w, bool quantized, const Q
     P.n = (int)q.n;
     P.out = q.out;
     P.big = w.big_leaves;
+    P.stack_depth = 3 * w.wide_depth + 4;
     if (kind == kSilhouette) {
         P.nodes = quantized ? w.qsnodes : w.snodes;
         P.prims = w.sils;
@@ -773,10 +1025,23 @@ hipError_t launch_dim(QueryKind kind, const WideView &w, bool quantized, const Q
         P.root = w.root_ref;
     }
     (void)ws;
-    void *buf = nullptr;
+    void *buf = nullptr, *dbuf = nullptr;
+    P.budget = 0;
     hipError_t e = sort_queries<D>(q.x, P.n, st, &P.perm, &buf);
+    if (e == hipSuccess && kind == kClosestPoint && 256 * (w.wide_depth + 1) <= kFront) {
+        e = hipMallocAsync(&dbuf, 256 + (size_t)P.n * sizeof(uint4), st);
+        if (e == hipSuccess) {
+            P.budget = kPacketBudget;
+            P.defer_count = (uint32_t *)dbuf;
+            P.deferred = (uint4 *)((uint8_t *)dbuf + 256);
+        }
+    }
     if (e == hipSuccess)
         e = quantized ? launch_traversal<D, true>(kind, P, st) : launch_traversal<D, false>(kind, P, st);
+    if (dbuf) {
+        hipError_t f = hipFreeAsync(dbuf, st);
+        if (e == hipSuccess) e = f;
+    }
     if (buf) {
         hipError_t f = hipFreeAsync(buf, st);
         if (e == hipSuccess) e = f;
@@ -806,8 +1071,9 @@ void EngineWorkspace::release() {
 }
 
 bool engine_supported(QueryKind kind, const WideView &w) {
-    if (kind == kSilhouette) return w.bin_depth + 2 <= kStackCap;
-    return 3 * w.wide_depth + 4 <= kStackCap;
+    if (kind == kSilhouette) return w.bin_depth + 2 <= kPacketDepth;
+    if (kind == kClosestPoint) return 3 * w.wide_depth + 4 <= kPacketDepth;
+    return 3 * w.wide_depth + 4 <= kRayStackMax;
 }
 
 hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q,

@@ -204,8 +204,12 @@ int gq_scene_destroy(gq_scene *scene) {
     return GQ_OK;
 }
 
-int gq_scene_create(const gq_structure *st, gq_scene **out) {
+int gq_scene_create(const gq_structure *st, gq_scene **out) { return gq_scene_create_ex(st, nullptr, out); }
+
+int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_scene **out) {
     if (!st || !out) return fail(GQ_ERR_INVALID, "null argument");
+    const int leaf_prims = (opt && opt->leaf_prims > 0) ? opt->leaf_prims : 2;
+    if (leaf_prims > 16) return fail(GQ_ERR_INVALID, "leaf_prims must be in 1..16");
     *out = nullptr;
     const HostStructure &h = st->h;
     std::unique_ptr<gq_scene> s(new (std::nothrow) gq_scene());
@@ -234,7 +238,7 @@ int gq_scene_create(const gq_structure *st, gq_scene **out) {
     // Engine layouts.
     WideHost wh;
     try {
-        build_wide(h, wh);
+        build_wide(h, wh, leaf_prims);
     } catch (const std::bad_alloc &) {
         return cleanup(fail(GQ_ERR_OOM, "host allocation failed while building wide nodes"));
     }
@@ -333,7 +337,7 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     if (!st || !n_bytes) return fail(GQ_ERR_INVALID, "null argument");
     WideHost wh;
     try {
-        build_wide(st->h, wh);
+        build_wide(st->h, wh, 2);
     } catch (const std::bad_alloc &) {
         return fail(GQ_ERR_OOM, "host allocation failed");
     }

@@ -401,14 +401,26 @@ template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_
     for (int k = 0; k < D; ++k) { out.scene_lo[k] = b[0].lo[k]; out.scene_hi[k] = b[0].hi[k]; }
     // primitives in leaf order
     const int ps = kPrimStride[D];
-    out.prims.resize((size_t)h.n_prims * (D == 3 ? 3 : 1));
+    out.prims.resize((size_t)h.n_prims * (D == 3 ? 4 : 1));
     out.prim_ids.resize((size_t)h.n_prims);
     for (int64_t s = 0; s < h.n_prims; ++s) {
         const float *r = h.primitive_data.data() + s * ps;
         if (D == 3) {
-            out.prims[3 * s + 0] = make_float4(r[0], r[1], r[2], 0.0f);
-            out.prims[3 * s + 1] = make_float4(r[3], r[4], r[5], 0.0f);
-            out.prims[3 * s + 2] = make_float4(r[6], r[7], r[8], 0.0f);
+            out.prims[4 * s + 0] = make_float4(r[0], r[1], r[2], 0.0f);
+            out.prims[4 * s + 1] = make_float4(r[3], r[4], r[5], 0.0f);
+            out.prims[4 * s + 2] = make_float4(r[6], r[7], r[8], 0.0f);
+            // supporting plane (unit normal, offset) for the closest-point prefilter: |n.x - o| is a
+            // lower bound on the distance to the triangle; a degenerate triangle gets n = 0 (no bound)
+            double e1[3], e2[3], n[3];
+            for (int k = 0; k < 3; ++k) { e1[k] = (double)r[3 + k] - r[k]; e2[k] = (double)r[6 + k] - r[k]; }
+            n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+            double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (l > 0 && std::isfinite(l)) { for (int k = 0; k < 3; ++k) n[k] /= l; }
+            else { n[0] = n[1] = n[2] = 0.0; }
+            double off = n[0] * r[0] + n[1] * r[1] + n[2] * r[2];
+            out.prims[4 * s + 3] = make_float4((float)n[0], (float)n[1], (float)n[2], (float)off);
         } else {
             out.prims[s] = make_float4(r[0], r[1], r[2], r[3]);
         }

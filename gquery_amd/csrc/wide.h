@@ -16,6 +16,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include <vector>
@@ -109,6 +110,20 @@ template <> struct alignas(64) SPairQ<2> {
     int32_t ref_cone[2];
 };
 static_assert(sizeof(SPairQ<3>) == 64 && sizeof(SPairQ<2>) == 64, "SPairQ is one line");
+// byte offsets the kernels decode (engine.hip load_pair / load_node4)
+static_assert(offsetof(SPairQ<3>, qlo) == 16 && offsetof(SPairQ<3>, qhi) == 22, "SPairQ<3> boxes");
+static_assert(offsetof(SPairQ<2>, qlo) == 12 && offsetof(SPairQ<2>, qhi) == 20, "SPairQ<2> boxes");
+static_assert(offsetof(SPairQ<3>, cone_axis) == 28 && offsetof(SPairQ<2>, cone_axis) == 28, "cone axis");
+static_assert(offsetof(SPairQ<3>, cone_cs) == 40 && offsetof(SPairQ<2>, cone_cs) == 40, "cone cs");
+static_assert(offsetof(SPairQ<3>, child) == 48 && offsetof(SPairQ<2>, child) == 48, "pair child");
+static_assert(offsetof(SPairQ<3>, ref_cone) == 56 && offsetof(SPairQ<2>, ref_cone) == 56, "pair ref");
+static_assert(offsetof(SPairF<3>, cone) == 48 && offsetof(SPairF<3>, child) == 88 && offsetof(SPairF<3>, ref_cone) == 96, "SPairF<3>");
+static_assert(offsetof(SPairF<2>, cone) == 32 && offsetof(SPairF<2>, child) == 72 && offsetof(SPairF<2>, ref_cone) == 80, "SPairF<2>");
+static_assert(offsetof(QNode<3>, qlo) == 16 && offsetof(QNode<3>, qhi) == 28 && offsetof(QNode<3>, child) == 40, "QNode<3>");
+static_assert(offsetof(QNode<2>, nchild) == 10 && offsetof(QNode<2>, qlo) == 12 && offsetof(QNode<2>, qhi) == 28 &&
+              offsetof(QNode<2>, child) == 44, "QNode<2>");
+static_assert(offsetof(FNode<3>, child) == 96 && offsetof(FNode<3>, nchild) == 112, "FNode<3>");
+static_assert(offsetof(FNode<2>, child) == 64 && offsetof(FNode<2>, nchild) == 80, "FNode<2>");
 
 // Reference normal cone (snch.cpp:102-154 values, float32) of a node whose cone test can cull
 // (0 <= half_angle <= pi/2); tested exactly as SNCHNode.overlaps does.
@@ -135,7 +150,7 @@ struct WideHost {
     std::vector<uint32_t> big_leaves;  // (first, count) pairs of leaves with more than 126 entries
 };
 
-void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims = 4);
+void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims);
 
 // Device scratch owned by a scene (Morton keys, permutation, radix-sort temp).  Grown on demand
 // with hipMalloc; reused by later calls (calls on one scene are ordered on their streams).

@@ -88,6 +88,13 @@ int gq_build_flat_silhouette_edges(const float *vertices, int64_t n_vertices, co
  * point and ray queries; an SNCH structure supports all three. */
 int gq_scene_create(const gq_structure *s, gq_scene **out);
 int gq_scene_destroy(gq_scene *scene);
+
+/* Engine layout options (no counterpart in the reference; chosen by the autotuner). */
+typedef struct gq_scene_options {
+    int32_t leaf_prims; /* max primitives per engine BVH4 leaf, 1..16; 0 = default (2) */
+    int32_t reserved[7];
+} gq_scene_options;
+int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
 /* info[0..7] = dim, has_snch, device bytes, n_nodes, n_wide_nodes, n_prims, n_silhouettes, 0 */
 int gq_scene_info(const gq_scene *scene, int64_t info[8]);
 

@@ -75,8 +75,15 @@ def test_bvh4_layout(name):
     assert len(q) == len(f) == n_nodes and root == 0
     n_prims = F.shape[0]
     assert np.array_equal(np.sort(ids), np.arange(n_prims))
-    pts = prims.reshape(n_prims, 3, 4)[:, :, :3] if D == 3 else prims.reshape(n_prims, 2, 2)
+    pts = prims.reshape(n_prims, 4, 4)[:, :3, :3] if D == 3 else prims.reshape(n_prims, 2, 2)
     np.testing.assert_array_equal(pts, V[F[ids]])
+    if D == 3:
+        # supporting plane: unit normal and offset, |n.x - o| <= distance for points on the triangle
+        pl = prims.reshape(n_prims, 4, 4)[:, 3, :].astype(np.float64)
+        good = np.linalg.norm(pl[:, :3], axis=1) > 0
+        for k in range(3):
+            res = np.abs((pts[:, k].astype(np.float64) * pl[:, :3]).sum(1) - pl[:, 3])
+            assert (res[good] < 1e-5).all()
     covered = np.zeros(n_prims, np.int64)
     seen = np.zeros(n_nodes, np.int64)
 
