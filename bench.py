@@ -78,6 +78,9 @@ def main():
                     help="query type to time (the headline metric is cp)")
     ap.add_argument("--n", type=int, default=N_QUERIES, help="queries per step")
     ap.add_argument("--subdiv", type=int, default=SUBDIV)
+    ap.add_argument("--shape", default="mesh", choices=["mesh", "polyline"],
+                    help="polyline: 2D star polyline with --segments segments")
+    ap.add_argument("--segments", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; queries shard by batch with no "
@@ -92,13 +95,20 @@ def main():
 
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    V, F = synth.displaced_icosphere(args.subdiv)
-    mesh = Mesh(V, F, device="cuda:0", leaf_prims=args.leaf_prims)
-    log(f"scene built in {time.perf_counter() - t0:.1f}s: {F.shape[0]} triangles")
+    dim = 3
+    if args.shape == "polyline":
+        from gquery_amd.shapes.polyline import Polyline
+        dim = 2
+        V, F = synth.star_polyline(args.segments)
+        mesh = Polyline(V, F, device="cuda:0", leaf_prims=args.leaf_prims)
+    else:
+        V, F = synth.displaced_icosphere(args.subdiv)
+        mesh = Mesh(V, F, device="cuda:0", leaf_prims=args.leaf_prims)
+    log(f"scene built in {time.perf_counter() - t0:.1f}s: {F.shape[0]} primitives")
     nq = args.n
-    x_host = synth.uniform_points(nq, 3)
+    x_host = synth.uniform_points(nq, dim)
     x = torch.from_numpy(x_host).cuda()
-    v = torch.from_numpy(synth.uniform_directions(nq, 3)).cuda()
+    v = torch.from_numpy(synth.uniform_directions(nq, dim)).cuda()
     torch.cuda.synchronize()
 
     method = args.method
@@ -128,11 +138,12 @@ def main():
     kern_ms, kern_launches, kern_name = _lib.profile_read()
     ms_per_step = ms_total / args.steps
     value = nq / (ms_per_step * 1e-3) / 1e6
-    log(f"{method} {args.query} s={args.subdiv} n={nq}: {ms_per_step:.3f} ms/step ({wall / args.steps * 1e3:.3f} ms wall), {value:.1f} Mq/s; "
+    log(f"{method} {args.shape} {args.query} s={args.subdiv} n={nq}: {ms_per_step:.3f} ms/step ({wall / args.steps * 1e3:.3f} ms wall), {value:.1f} Mq/s; "
         f"traversal kernel {kern_name} {kern_ms / max(kern_launches, 1):.3f} ms/launch")
 
-    if args.query != "cp" or args.subdiv != SUBDIV or nq != N_QUERIES:
-        print(json.dumps({"query": args.query, "method": method, "subdiv": args.subdiv, "n": nq,
+    if args.query != "cp" or args.subdiv != SUBDIV or nq != N_QUERIES or dim != 3:
+        print(json.dumps({"shape": args.shape, "query": args.query, "method": method,
+                          "prims": int(F.shape[0]), "n": nq,
                           "ms_per_step": ms_per_step, "Mq_per_s": value, "kernel_ms": kern_ms / max(kern_launches, 1)}))
         return
 

@@ -276,6 +276,20 @@ template <int D> __device__ __forceinline__ void packet_centroid(bool live, cons
     for (int k = 0; k < D; ++k) cen[k] = uniformf(wave_sum(live ? pc[k] : 0.0f) / nl);
 }
 
+// Hands the live lanes of a wavefront that exceeded its step budget to the wave-per-query pass:
+// query index, current bound and best slot so far, appended with one atomic per wavefront.
+__device__ __forceinline__ void defer_wave(const EngineParams &P, bool live, uint32_t qi, float bound, int slot) {
+    const unsigned long long m = __ballot(live);
+    const uint32_t nlive = (uint32_t)__popcll(m);
+    // every lane takes part in the atomic (lane 0 adds the count) so no lane-dependent branch
+    // precedes the return
+    const uint32_t base = uniform(atomicAdd(P.defer_count, (threadIdx.x & 63) == 0 ? nlive : 0u));
+    if (live) {
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
+        P.deferred[base + rank] = make_uint4(qi, __float_as_uint(bound), (uint32_t)slot, 0u);
+    }
+}
+
 template <int D, bool Q>
 __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
     using V = typename VT<D>::T;
@@ -301,15 +315,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
             // Too much work for one wavefront (queries deep inside a closed surface see most of it
             // at nearly one distance): hand the queries, with the answer so far as the starting
             // bound, to the wave-per-query pass so they spread over many wavefronts.
-            const unsigned long long m = __ballot(live);
-            const uint32_t nlive = (uint32_t)__popcll(m);
-            uint32_t base = 0;
-            if ((threadIdx.x & 63) == 0) base = atomicAdd(P.defer_count, nlive);
-            base = uniform(__shfl(base, 0, 64));
-            if (live) {
-                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
-                P.deferred[base + rank] = make_uint4(qi, __float_as_uint(best), (uint32_t)best_slot, 0u);
-            }
+            defer_wave(P, live, qi, best, best_slot);
             return;
         }
         if (!ref_is_leaf(cur)) {
@@ -407,9 +413,10 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
     const int lane = threadIdx.x & 63;
     const uint32_t total = uniform(P.defer_count[0]);
     for (;;) {
-        uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(P.defer_count + 1, 1u);
-        k = uniform(__shfl(k, 0, 64));
+        // All lanes take part in the fetch (lane 0 adds 1, the others 0) and later all lanes store
+        // the same record: no lane-dependent branch inside this loop, so the compiler cannot split
+        // the wavefront across iterations.
+        const uint32_t k = uniform(atomicAdd(P.defer_count + 1, lane == 0 ? 1u : 0u));
         if (k >= total) break;
         const uint4 item = P.deferred[k];
         const uint32_t qi = uniform(item.x);
@@ -427,9 +434,8 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
             uint2 e = make_uint2(0u, __float_as_uint(kInf));
             if (lane < take) e = front[size - take + lane];
             size -= take;
-            uint32_t child[4];
-            float ckey[4];
-            int nh = 0;
+            uint2 ent[4] = {};
+            bool hit[4] = {false, false, false, false};
             float cand = kInf;
             int cand_slot = -1;
             if (lane < take && __uint_as_float(e.y) <= bound2) {
@@ -444,11 +450,8 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
                             const float o = fmaxf(fmaxf(nd.lo[c][j] - pc[c], pc[c] - nd.hi[c][j]), 0.0f);
                             d2 = fmaf(o, o, d2);
                         }
-                        if (j < nd.n && d2 <= bound2) {
-                            child[nh] = nd.child[j];
-                            ckey[nh] = d2;
-                            ++nh;
-                        }
+                        hit[j] = j < nd.n && d2 <= bound2;
+                        ent[j] = make_uint2(nd.child[j], __float_as_uint(d2));
                     }
                 } else {
                     uint32_t first, count;
@@ -494,13 +497,17 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
             }
             // append the opened children
             int tot;
-            const int off = wave_exclusive_scan(nh, tot);
+            const int nh = (int)hit[0] + (int)hit[1] + (int)hit[2] + (int)hit[3];
+            int pos = size + wave_exclusive_scan(nh, tot);
             tot = (int)uniform((uint32_t)tot);
-            for (int j = 0; j < nh; ++j)
-                if (size + off + j < kFront) front[size + off + j] = make_uint2(child[j], __float_as_uint(ckey[j]));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (hit[j] && pos < kFront) front[pos] = ent[j];
+                pos += (int)hit[j];
+            }
             size = size + tot < kFront ? size + tot : kFront;
         }
-        if (lane == 0) cp_record<D>(P, qi, p, best_slot);
+        cp_record<D>(P, qi, p, best_slot);
     }
 }
 
@@ -760,8 +767,14 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
     if (!(d_root <= r)) r = -1.0f;
     if (__ballot(r >= 0.0f) != 0ull) {
         int sp = 0;
+        int steps = 0;
         uint32_t cur = uniform(P.root);
         for (;;) {
+            if (P.budget > 0 && ++steps > P.budget) {
+                // interior queries see no silhouette and little culling: finish them one per wave
+                defer_wave(P, live, qi, r, found ? best_slot : -1);
+                return;
+            }
             if (!ref_is_leaf(cur)) {
                 PairChild c[2];
                 load_pair<D, Q>(P.nodes, cur, c);
@@ -849,6 +862,148 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
     if (o.p) gqd::st(o.p + (size_t)D * qi, best_p);
     if (o.d) o.d[qi] = best_d;
     if (o.prim_id) o.prim_id[qi] = P.ids[best_slot];
+}
+
+// ---------------------------------------------------------------------------------------------
+// closest silhouette, wave per query: finishes the queries the packet pass deferred, with the same
+// node tests (reference box and cone test, exact cone cull).  Frontier as in the closest-point
+// pass: at most 128 entries per binary level.
+constexpr int kFrontSil = 3072;
+
+template <int D, bool Q>
+__global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) {
+    using V = typename VT<D>::T;
+    __shared__ uint2 front_all[kWaveBlock / 64][kFrontSil];
+    uint2 *front = front_all[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const uint32_t total = uniform(P.defer_count[0]);
+    for (;;) {
+        // All lanes take part in the fetch (lane 0 adds 1, the others 0) and later all lanes store
+        // the same record: no lane-dependent branch inside this loop, so the compiler cannot split
+        // the wavefront across iterations.
+        const uint32_t k = uniform(atomicAdd(P.defer_count + 1, lane == 0 ? 1u : 0u));
+        if (k >= total) break;
+        const uint4 item = P.deferred[k];
+        const uint32_t qi = uniform(item.x);
+        float r = uniformf(__uint_as_float(item.y));
+        int best_slot = (int)uniform(item.z);
+        const V x = ldv<D>(P.x + (size_t)D * qi);
+        float xc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < D; ++j) xc[j] = comp<D>(x, j);
+        int size = 0;
+        if (r >= 0.0f) {
+            front[0] = make_uint2(P.root, 0u);
+            size = 1;
+        }
+        while (size > 0) {
+            const int take = size < 64 ? size : 64;
+            uint2 e = make_uint2(0u, __float_as_uint(kInf));
+            if (lane < take) e = front[size - take + lane];
+            size -= take;
+            // opened children live in fixed slots (no dynamically indexed register arrays)
+            uint2 ch0 = make_uint2(0u, 0u), ch1 = make_uint2(0u, 0u);
+            bool h0 = false, h1 = false;
+            float cand = kInf;
+            int cand_slot = -1;
+            if (lane < take && __uint_as_float(e.y) <= r) {
+                if (!ref_is_leaf(e.x)) {
+                    PairChild c[2];
+                    load_pair<D, Q>(P.nodes, e.x, c);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        float dm, dx;
+                        box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
+                        bool h = dm <= r;
+                        if (h && c[j].ref_cone >= 0) {
+                            const RefCone &rc = P.ref_cones[c[j].ref_cone];
+                            h = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dm);
+                        }
+                        if (h && exact_cull<D>(c[j], xc)) h = false;
+                        const uint2 ent = make_uint2(c[j].child, __float_as_uint(dm));
+                        if (j == 0) { h0 = h; ch0 = ent; }
+                        else { h1 = h; ch1 = ent; }
+                    }
+                } else {
+                    uint32_t first, count;
+                    leaf_range(e.x, P.big, first, count);
+                    for (uint32_t s = first; s < first + count; ++s) {
+                        if constexpr (D == 3) {
+                            const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1], s2 = P.prims[3 * s + 2];
+                            const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
+                            V3 pt;
+                            const float dd = closest_point_segment(x, a, b, pt);
+                            if (dd < r && (dd < cand || (dd == cand && (int)s < cand_slot))) {
+                                const V3 n0 = v3(s1.z, s1.w, s2.x), n1 = v3(s2.y, s2.z, s2.w);
+                                const V3 vd = pt - x;
+                                if (dot(vd, n0) * dot(vd, n1) < 0.0f) {
+                                    cand = dd;
+                                    cand_slot = (int)s;
+                                }
+                            }
+                        } else {
+                            const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
+                            const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
+                            const float dd = norm(x - b);
+                            if (dd < r && (dd < cand || (dd == cand && (int)s < cand_slot)) &&
+                                cross2(b - a, x - a) * cross2(cc - b, x - b) < 0.0f) {
+                                cand = dd;
+                                cand_slot = (int)s;
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const float oc = __shfl_xor(cand, o, 64);
+                const int os = __shfl_xor(cand_slot, o, 64);
+                if (oc < cand || (oc == cand && os < cand_slot)) {
+                    cand = oc;
+                    cand_slot = os;
+                }
+            }
+            cand = uniformf(cand);
+            cand_slot = (int)uniform((uint32_t)cand_slot);
+            if (cand < r) {
+                r = cand;
+                best_slot = cand_slot;
+            }
+            int tot;
+            const int nh = (int)h0 + (int)h1;
+            int pos = size + wave_exclusive_scan(nh, tot);
+            tot = (int)uniform((uint32_t)tot);
+            if (h0 && pos < kFrontSil) front[pos] = ch0;
+            pos += (int)h0;
+            if (h1 && pos < kFrontSil) front[pos] = ch1;
+            size = size + tot < kFrontSil ? size + tot : kFrontSil;
+        }
+        {
+            const gq_records &o = P.out;
+            if (best_slot < 0) {
+                if (o.valid) o.valid[qi] = 0;
+                if (o.p) for (int j = 0; j < D; ++j) o.p[(size_t)D * qi + j] = 0.0f;
+                if (o.d) o.d[qi] = 0.0f;
+                if (o.prim_id) o.prim_id[qi] = 0;
+            } else {
+                const uint32_t s = (uint32_t)best_slot;
+                V bp;
+                float bd;
+                if constexpr (D == 3) {
+                    const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1];
+                    bd = closest_point_segment(x, v3(s0.x, s0.y, s0.z), v3(s0.w, s1.x, s1.y), bp);
+                } else {
+                    const float4 s0 = P.prims[2 * s];
+                    bp = v2(s0.z, s0.w);
+                    bd = norm(x - bp);
+                }
+                if (o.valid) o.valid[qi] = 1;
+                if (o.p) gqd::st(o.p + (size_t)D * qi, bp);
+                if (o.d) o.d[qi] = bd;
+                if (o.prim_id) o.prim_id[qi] = P.ids[s];
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -960,7 +1115,10 @@ hipError_t launch_traversal(QueryKind kind, const EngineParams &P, hipStream_t s
             if (P.budget > 0)
                 hipLaunchKernelGGL((k_closest_point_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, P);
         } else {
+            if (P.budget > 0) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, P.defer_count, 2);
             hipLaunchKernelGGL((k_silhouette<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
+            if (P.budget > 0)
+                hipLaunchKernelGGL((k_silhouette_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, P);
         }
     }
     prof_end(st, names[D - 2][kind]);
@@ -1028,10 +1186,12 @@ hipError_t launch_dim(QueryKind kind, const WideView &w, bool quantized, const Q
     void *buf = nullptr, *dbuf = nullptr;
     P.budget = 0;
     hipError_t e = sort_queries<D>(q.x, P.n, st, &P.perm, &buf);
-    if (e == hipSuccess && kind == kClosestPoint && 256 * (w.wide_depth + 1) <= kFront) {
+    const bool defer = (kind == kClosestPoint && 256 * (w.wide_depth + 1) <= kFront) ||
+                       (kind == kSilhouette && 128 * (w.bin_depth + 2) <= kFrontSil);
+    if (e == hipSuccess && defer && w.packet_budget >= 0) {
         e = hipMallocAsync(&dbuf, 256 + (size_t)P.n * sizeof(uint4), st);
         if (e == hipSuccess) {
-            P.budget = kPacketBudget;
+            P.budget = w.packet_budget > 0 ? w.packet_budget : kPacketBudget;
             P.defer_count = (uint32_t *)dbuf;
             P.deferred = (uint4 *)((uint8_t *)dbuf + 256);
         }

@@ -246,6 +246,7 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     wv.dim = h.dim;
     wv.width = wh.width;
     wv.wide_depth = wh.wide_depth;
+    wv.packet_budget = opt ? opt->packet_budget : 0;
     wv.bin_depth = h.max_depth;
     wv.n_nodes = (int64_t)wh.n_nodes;
     wv.n_snch_nodes = (int64_t)wh.n_snch_nodes;

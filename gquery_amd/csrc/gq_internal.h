@@ -41,6 +41,7 @@ struct WideView {
     uint32_t root_ref, sroot_ref;
     float sroot_lo[3], sroot_hi[3];
     float scene_lo[3], scene_hi[3];
+    int packet_budget;        // 0 = default, < 0 = no deferral to the wave-per-query pass
 };
 
 struct QueryArgs {

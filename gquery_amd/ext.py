@@ -43,11 +43,13 @@ class _Structure:
         _lib.check(fn(self._dim, v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0],
                       int(max_prims_in_node), C.byref(h)), type(self).__name__)
         self._h = h
+        # held by the object: module globals may already be gone when it is collected at exit
+        self._free = _lib.lib().gq_structure_free
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _lib.lib().gq_structure_free(h)
+            self._free(h)
             self._h = None
 
     @property

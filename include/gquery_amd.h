@@ -91,8 +91,10 @@ int gq_scene_destroy(gq_scene *scene);
 
 /* Engine layout options (no counterpart in the reference; chosen by the autotuner). */
 typedef struct gq_scene_options {
-    int32_t leaf_prims; /* max primitives per engine BVH4 leaf, 1..16; 0 = default (2) */
-    int32_t reserved[7];
+    int32_t leaf_prims;    /* max primitives per engine BVH4 leaf, 1..16; 0 = default (2) */
+    int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
+                              move to the wave-per-query pass; 0 = default (3000), < 0 = never */
+    int32_t reserved[6];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
 /* info[0..7] = dim, has_snch, device bytes, n_nodes, n_wide_nodes, n_prims, n_silhouettes, 0 */

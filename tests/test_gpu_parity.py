@@ -149,3 +149,30 @@ def test_edge_cases(gpu, method, dim):
         got = _np(shape.closest_silhouette_snch(x, torch.as_tensor(r_max), method=method), ("valid", "prim_id", "d", "p"))
         res = parity.summarize("sil r_max", got, ref, None, vec_fields=())
         assert res["valid_mismatch"] == 0 and res["max_err_d"] < 1e-5, res
+
+
+@pytest.mark.parametrize("budget", [1, 6])
+@pytest.mark.parametrize("name", ["star4096", "ico5"])
+@pytest.mark.parametrize("kind", ["cp", "sil"])
+@pytest.mark.parametrize("method", ["engine_fp32", "engine"])
+def test_wave_per_query_pass(gpu, budget, name, kind, method):
+    """A tiny packet step budget sends (nearly) every query through the wave-per-query pass, which
+    starts from the packet pass's partial answer; results must meet the same rule as above."""
+    from gquery_amd.shapes.mesh import Mesh
+    from gquery_amd.shapes.polyline import Polyline
+    V, F, base = scene(name)
+    dim = V.shape[1]
+    shape = (Polyline if dim == 2 else Mesh)(V, F, device="cuda:0", packet_budget=budget)
+    x = synth.uniform_points(3000, dim, seed=31)
+    for r_max in (float("inf"), 0.25):
+        if kind == "cp" and r_max != float("inf"):
+            continue
+        ref = _oracle(kind, dim, shape.snch, x, None, r_max)
+        got = _np(_query(shape, kind, method, x, None, r_max))
+        gap = _gap_from_bruteforce(kind, dim, shape.snch, V, F, x, None) if r_max == float("inf") else None
+        vec = ("p", "n") if kind == "cp" else ("p",)
+        res = parity.summarize(f"{name}/{kind}/{method}/budget{budget}/r{r_max}", got, ref, gap, vec_fields=vec)
+        if gap is None:   # ids and points only defined up to ties here: valid flags and distances
+            assert res["valid_mismatch"] == 0 and res["max_err_d"] < parity.TOL_REF, res
+        else:
+            assert res["ok"], res
