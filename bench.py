@@ -66,6 +66,51 @@ def cpu_baseline(scene, x, budget_s=10.0):
                       f"OpenMP over {O.num_threads()} host threads, {dt:.1f} s"}
 
 
+def _pmc_table(d):
+    """{kernel name: {counter: mean value per dispatch}} from rocprofv3 csv counter output."""
+    import csv
+    acc = {}
+    for f in Path(d).rglob("*counter_collection.csv"):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                k = (row["Kernel_Name"], row["Counter_Name"])
+                acc.setdefault(k, {}).setdefault(row.get("Dispatch_Id", len(acc)), 0.0)
+                acc[k][row.get("Dispatch_Id")] += float(row["Counter_Value"])
+    out = {}
+    for (kn, cn), per in acc.items():
+        out.setdefault(kn, {})[cn] = sum(per.values()) / max(len(per), 1)
+    return out
+
+
+def pmc_summary(args):
+    """Per-launch HBM traffic of the traversal kernels, each counter scaled by what it reads for
+    the elementwise copy of known size (copy_mb MiB read + copy_mb MiB written)."""
+    fetch, write = _pmc_table(args.pmc_summary[0]), _pmc_table(args.pmc_summary[1])
+    copy_bytes = float(args.copy_mb << 20)
+    ck = [k for k in fetch if "elementwise" in k.lower() or "mul" in k.lower()]
+    if not ck:
+        raise SystemExit("no elementwise copy kernel in the counter output (run with --copy-mb)")
+    ck = max(ck, key=lambda k: fetch[k].get("FETCH_SIZE", 0.0))
+    f_scale = copy_bytes / (fetch[ck]["FETCH_SIZE"] * 1024.0)
+    w_scale = copy_bytes / (write[ck]["WRITE_SIZE"] * 1024.0)
+    trav = [k for k in fetch if k.startswith("void gqa::") or "gqa::" in k]
+    rows, tot_f, tot_w = [], 0.0, 0.0
+    for k in sorted(trav):
+        if not any(t in k for t in ("k_closest_point", "k_intersect", "k_silhouette", "k_zero")):
+            continue
+        fb = fetch[k].get("FETCH_SIZE", 0.0) * 1024.0 * f_scale
+        wb = write.get(k, {}).get("WRITE_SIZE", 0.0) * 1024.0 * w_scale
+        rows.append((k, fb, wb))
+        tot_f += fb
+        tot_w += wb
+    tag = f"traffic_cp3d_s7_1M_{args.method}"
+    res = {"hbm_bytes_per_launch": tot_f + tot_w, "fetch_bytes": tot_f, "write_bytes": tot_w,
+           "fetch_scale": f_scale, "write_scale": w_scale, "copy_kernel": ck[:120],
+           "kernels": {k[:120]: {"fetch_bytes": fb, "write_bytes": wb} for k, fb, wb in rows}}
+    (ROOT / "profiles" / f"{tag}.json").write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +119,8 @@ def main():
     ap.add_argument("--method", default="engine", choices=["engine", "engine_fp32", "plain"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--leaf-prims", type=int, default=0, help="engine BVH4 leaf size (0 = default)")
+    ap.add_argument("--packet-budget", type=int, default=0,
+                    help="packet steps per wavefront before deferral (0 = default, -1 = never)")
     ap.add_argument("--query", default="cp", choices=["cp", "ray", "sil"],
                     help="query type to time (the headline metric is cp)")
     ap.add_argument("--n", type=int, default=N_QUERIES, help="queries per step")
@@ -81,7 +128,16 @@ def main():
     ap.add_argument("--shape", default="mesh", choices=["mesh", "polyline"],
                     help="polyline: 2D star polyline with --segments segments")
     ap.add_argument("--segments", type=int, default=65536)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after timing, run 3 elementwise float copies of this many MiB (the known-size "
+                         "kernel that calibrates FETCH_SIZE / WRITE_SIZE counter runs)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU: turn two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; csv) of "
+                         "`bench.py --copy-mb M` into profiles/traffic_<workload>_<method>.{json,md}")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; queries shard by batch with no "
                                    "exchange, so multi-GPU numbers are not measured here"}))
@@ -100,10 +156,10 @@ def main():
         from gquery_amd.shapes.polyline import Polyline
         dim = 2
         V, F = synth.star_polyline(args.segments)
-        mesh = Polyline(V, F, device="cuda:0", leaf_prims=args.leaf_prims)
+        mesh = Polyline(V, F, device="cuda:0", leaf_prims=args.leaf_prims, packet_budget=args.packet_budget)
     else:
         V, F = synth.displaced_icosphere(args.subdiv)
-        mesh = Mesh(V, F, device="cuda:0", leaf_prims=args.leaf_prims)
+        mesh = Mesh(V, F, device="cuda:0", leaf_prims=args.leaf_prims, packet_budget=args.packet_budget)
     log(f"scene built in {time.perf_counter() - t0:.1f}s: {F.shape[0]} primitives")
     nq = args.n
     x_host = synth.uniform_points(nq, dim)
@@ -135,10 +191,17 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = ev0.elapsed_time(ev1)
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb << 18, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     kern_ms, kern_launches, kern_name = _lib.profile_read()
     ms_per_step = ms_total / args.steps
     value = nq / (ms_per_step * 1e-3) / 1e6
-    log(f"{method} {args.shape} {args.query} s={args.subdiv} n={nq}: {ms_per_step:.3f} ms/step ({wall / args.steps * 1e3:.3f} ms wall), {value:.1f} Mq/s; "
+    log(f"{method} {args.shape} {args.query} s={args.subdiv} n={nq} leaf={args.leaf_prims} budget={args.packet_budget}: {ms_per_step:.3f} ms/step ({wall / args.steps * 1e3:.3f} ms wall), {value:.1f} Mq/s; "
         f"traversal kernel {kern_name} {kern_ms / max(kern_launches, 1):.3f} ms/launch")
 
     if args.query != "cp" or args.subdiv != SUBDIV or nq != N_QUERIES or dim != 3:

@@ -29,9 +29,11 @@ using namespace gqd;
 
 constexpr int kBlock = 256;     // packet kernels: 4 wavefronts per block
 constexpr int kRayBlock = 64;   // lane-per-query ray kernel: one wavefront per block
-constexpr int kRayStackMax = 96;
+constexpr int kRayStackMax = 96;  // lane-stack entries: 96 * 64 lanes * 8 B = 48 KB of LDS per block
 constexpr int kWavePassBlocks = 1024;  // persistent grid of the wave-per-query pass (4 per CU)
-constexpr int kPacketBudget = 3000;    // packet steps per wavefront before its queries are deferred  // lane-stack entries: 96 * 64 lanes * 8 B = 48 KB of LDS per block
+// Packet steps per wavefront before its queries are deferred to the wave-per-query pass, by query
+// kind and dimension (measured optimum for 4-primitive leaves; a scene option overrides it).
+constexpr int kPacketBudgetCp3 = 1500, kPacketBudgetCp2 = 1000, kPacketBudgetSil = 1000;
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
 
@@ -1020,14 +1022,12 @@ __global__ void k_zero_u32(uint32_t *b, int n) {
     for (int t = threadIdx.x; t < n; t += blockDim.x) b[t] = 0u;
 }
 
-__global__ void k_bounds_init(uint32_t *b) {
-    const int t = threadIdx.x;
-    if (t < 3) b[t] = 0xFFFFFFFFu;        // min
-    else if (t < 6) b[t] = 0u;            // max
-}
+constexpr int kBoundsBlocks = 256;
 
+// Per-block partial bounds (order-preserving uint encoding) into part[block][6]; no atomics, so
+// the result does not depend on scheduling and the pass can be recorded in a graph.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_bounds(const float *x, int n, uint32_t *b) {
+__global__ void __launch_bounds__(kBlock) k_bounds(const float *x, int n, uint32_t *part) {
     uint32_t mn[D], mx[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) { mn[k] = 0xFFFFFFFFu; mx[k] = 0u; }
@@ -1049,13 +1049,29 @@ __global__ void __launch_bounds__(kBlock) k_bounds(const float *x, int n, uint32
             mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ uint32_t red[kBlock / 64][6];
+    const int w = threadIdx.x >> 6;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            atomicMin(&b[k], mn[k]);
-            atomicMax(&b[3 + k], mx[k]);
-        }
+    for (int k = 0; k < 3; ++k) {
+        red[w][k] = k < D ? mn[k] : 0xFFFFFFFFu;   // every lane stores the same wave value
+        red[w][3 + k] = k < D ? mx[k] : 0u;
     }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t v = red[0][k];
+        for (int j = 1; j < kBlock / 64; ++j) v = k < 3 ? min(v, red[j][k]) : max(v, red[j][k]);
+        part[blockIdx.x * 6 + k] = v;
+    }
+}
+
+// Final bounds from the per-block partials: b[0..2] = min, b[3..5] = max.
+__global__ void k_bounds_final(const uint32_t *part, int nblocks, uint32_t *b) {
+    const int k = threadIdx.x;
+    if (k >= 6) return;
+    uint32_t v = k < 3 ? 0xFFFFFFFFu : 0u;
+    for (int j = 0; j < nblocks; ++j) v = k < 3 ? min(v, part[j * 6 + k]) : max(v, part[j * 6 + k]);
+    b[k] = v;
 }
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
@@ -1139,16 +1155,18 @@ hipError_t sort_queries(const float *x, int n, hipStream_t st, const uint32_t **
                                              D == 3 ? 30 : 32, st);
     if (e != hipSuccess) return e;
     const size_t nb = align256((size_t)n * 4);
-    const size_t need = 256 + 4 * nb + align256(temp);
+    const size_t need = 256 + align256(kBoundsBlocks * 6 * 4) + 4 * nb + align256(temp);
     if ((e = hipMallocAsync(buf, need, st)) != hipSuccess) return e;
     uint8_t *base = (uint8_t *)*buf;
     uint32_t *bounds = (uint32_t *)base;
-    uint32_t *k_in = (uint32_t *)(base + 256), *k_out = (uint32_t *)(base + 256 + nb);
-    uint32_t *v_in = (uint32_t *)(base + 256 + 2 * nb), *v_out = (uint32_t *)(base + 256 + 3 * nb);
-    void *tmp = base + 256 + 4 * nb;
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, st, bounds);
-    const unsigned rgrid = (unsigned)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_bounds<D>), dim3(rgrid), dim3(kBlock), 0, st, x, n, bounds);
+    uint32_t *part = (uint32_t *)(base + 256);
+    base += 256 + align256(kBoundsBlocks * 6 * 4);
+    uint32_t *k_in = (uint32_t *)base, *k_out = (uint32_t *)(base + nb);
+    uint32_t *v_in = (uint32_t *)(base + 2 * nb), *v_out = (uint32_t *)(base + 3 * nb);
+    void *tmp = base + 4 * nb;
+    const int rgrid = (int)std::min<int64_t>(kBoundsBlocks, (n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_bounds<D>), dim3(rgrid), dim3(kBlock), 0, st, x, n, part);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, part, rgrid, bounds);
     hipLaunchKernelGGL((k_morton<D>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, x, n, bounds, k_in, v_in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = rocprim::radix_sort_pairs(tmp, temp, k_in, k_out, v_in, v_out, (size_t)n, 0, D == 3 ? 30 : 32, st);
@@ -1191,7 +1209,10 @@ hipError_t launch_dim(QueryKind kind, const WideView &w, bool quantized, const Q
     if (e == hipSuccess && defer && w.packet_budget >= 0) {
         e = hipMallocAsync(&dbuf, 256 + (size_t)P.n * sizeof(uint4), st);
         if (e == hipSuccess) {
-            P.budget = w.packet_budget > 0 ? w.packet_budget : kPacketBudget;
+            P.budget = w.packet_budget > 0 ? w.packet_budget
+                       : kind == kSilhouette  ? kPacketBudgetSil
+                       : D == 3               ? kPacketBudgetCp3
+                                              : kPacketBudgetCp2;
             P.defer_count = (uint32_t *)dbuf;
             P.deferred = (uint4 *)((uint8_t *)dbuf + 256);
         }

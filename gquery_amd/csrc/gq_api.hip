@@ -208,7 +208,7 @@ int gq_scene_create(const gq_structure *st, gq_scene **out) { return gq_scene_cr
 
 int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_scene **out) {
     if (!st || !out) return fail(GQ_ERR_INVALID, "null argument");
-    const int leaf_prims = (opt && opt->leaf_prims > 0) ? opt->leaf_prims : 2;
+    const int leaf_prims = (opt && opt->leaf_prims > 0) ? opt->leaf_prims : kDefaultLeafPrims;
     if (leaf_prims > 16) return fail(GQ_ERR_INVALID, "leaf_prims must be in 1..16");
     *out = nullptr;
     const HostStructure &h = st->h;
@@ -338,7 +338,7 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     if (!st || !n_bytes) return fail(GQ_ERR_INVALID, "null argument");
     WideHost wh;
     try {
-        build_wide(st->h, wh, 2);
+        build_wide(st->h, wh, kDefaultLeafPrims);
     } catch (const std::bad_alloc &) {
         return fail(GQ_ERR_OOM, "host allocation failed");
     }

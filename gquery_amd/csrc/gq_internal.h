@@ -22,6 +22,8 @@ struct PlainView {
 
 // Engine layout (built on the host by wide.cpp, uploaded by the API layer).
 struct RefCone;
+constexpr int kDefaultLeafPrims = 4;  // engine BVH4 leaf size when the scene options give none
+
 struct WideView {
     int dim;
     int width;                // children per node (4)

@@ -91,9 +91,10 @@ int gq_scene_destroy(gq_scene *scene);
 
 /* Engine layout options (no counterpart in the reference; chosen by the autotuner). */
 typedef struct gq_scene_options {
-    int32_t leaf_prims;    /* max primitives per engine BVH4 leaf, 1..16; 0 = default (2) */
+    int32_t leaf_prims;    /* max primitives per engine BVH4 leaf, 1..16; 0 = default (4) */
     int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
-                              move to the wave-per-query pass; 0 = default (3000), < 0 = never */
+                              move to the wave-per-query pass; 0 = default (closest point 1500 in
+                              3D / 1000 in 2D, silhouettes 1000), < 0 = never */
     int32_t reserved[6];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
