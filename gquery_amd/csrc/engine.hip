@@ -53,6 +53,7 @@ struct EngineParams {
     int budget;               // packet steps before a wave hands its queries to the wave-per-query pass (0 = never)
     uint32_t *defer_count;    // [0] deferred queries, [1] work counter of the wave-per-query pass
     uint4 *deferred;          // (query index, best distance bits, best slot, 0)
+    int sil_lane;             // silhouettes: lane-per-query kernel instead of packet + wave pass
     gq_records out;
 };
 
@@ -266,6 +267,14 @@ __device__ __forceinline__ void cp_record(const EngineParams &P, uint32_t qi, co
     }
     if (o.prim_id) o.prim_id[qi] = P.ids[s];
     if (o.valid) o.valid[qi] = 1;
+}
+
+// Silhouette-edge predicate on an engine record (s1, s2 hold n0 and n1): a boundary edge is a
+// silhouette everywhere (silhouette_edge.py:54), else is_silhouette_edge (silhouette_edge.py:7-12) on the view
+// vector from the query to the nearest point of the edge.
+__device__ __forceinline__ bool sil_edge_test(const V3 &vd, const float4 &s1, const float4 &s2) {
+    if (__float_as_uint(s2.y) == kSilAlwaysTag) return true;
+    return dot(vd, v3(s1.z, s1.w, s2.x)) * dot(vd, v3(s2.y, s2.z, s2.w)) < 0.0f;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -747,7 +756,10 @@ template <int D, bool Q>
 __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
     using V = typename VT<D>::T;
     __shared__ uint32_t wstack[kBlock / 64][kPacketDepth];
+    __shared__ unsigned long long wmask[kBlock / 64][kPacketDepth];
     uint32_t *wst = wstack[threadIdx.x >> 6];
+    unsigned long long *wmk = wmask[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < P.n;
     const uint32_t qi = live ? query_index<D>(P, i) : 0u;
@@ -767,7 +779,12 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
     box_distance(ldv<D>(P.root_lo), ldv<D>(P.root_hi), x, d_root, d_unused);
     // a lane whose query misses the root box takes no part (reference: snch.py:111-112)
     if (!(d_root <= r)) r = -1.0f;
-    if (__ballot(r >= 0.0f) != 0ull) {
+    // `act`: this lane's own reference-order traversal reaches the current node.  Children are
+    // opened for the whole wave when any lane needs them, but a lane only tests the leaves it
+    // would reach itself, so every query sees exactly the silhouettes the reference sees (the
+    // reference cone test is not conservative for every mesh, so extra leaves could change it).
+    bool act = r >= 0.0f;
+    if (__ballot(act) != 0ull) {
         int sp = 0;
         int steps = 0;
         uint32_t cur = uniform(P.root);
@@ -780,19 +797,22 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
             if (!ref_is_leaf(cur)) {
                 PairChild c[2];
                 load_pair<D, Q>(P.nodes, cur, c);
-                bool any[2];
+                bool any[2], hh[2];
+                unsigned long long hm[2];
                 float key[2];
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     float dm, dx;
                     box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
-                    bool h = dm <= r;
+                    bool h = act && dm <= r;
                     if (h && c[j].ref_cone >= 0) {
                         const RefCone &rc = P.ref_cones[c[j].ref_cone];
                         h = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dm);
                     }
                     if (h && exact_cull<D>(c[j], xc)) h = false;
-                    any[j] = __ballot(h) != 0ull;
+                    hh[j] = h;
+                    hm[j] = __ballot(h);
+                    any[j] = hm[j] != 0ull;
                     float c2 = 0.0f;
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
@@ -803,12 +823,18 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                 }
                 if (any[0] && any[1]) {
                     const int near = key[1] < key[0] ? 1 : 0;
-                    if (sp < kPacketDepth) wst[sp++] = c[1 - near].child;
+                    if (sp < kPacketDepth) {
+                        wst[sp] = c[1 - near].child;
+                        wmk[sp] = hm[1 - near];
+                        ++sp;
+                    }
                     cur = uniform(c[near].child);
+                    act = near ? hh[1] : hh[0];
                     continue;
                 }
                 if (any[0] || any[1]) {
                     cur = uniform(any[0] ? c[0].child : c[1].child);
+                    act = any[0] ? hh[0] : hh[1];
                     continue;
                 }
             } else {
@@ -822,10 +848,8 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                         const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
                         V3 pt;
                         const float dd = closest_point_segment(x, a, b, pt);
-                        if (dd < r) {
-                            const V3 n0 = v3(s1.z, s1.w, s2.x), n1 = v3(s2.y, s2.z, s2.w);
-                            const V3 vd = pt - x;
-                            if (dot(vd, n0) * dot(vd, n1) < 0.0f) {
+                        if (act && dd < r) {
+                            if (sil_edge_test(pt - x, s1, s2)) {
                                 r = dd;
                                 found = true;
                                 best_p = pt;
@@ -837,7 +861,7 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                         const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
                         const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
                         const float dd = norm(x - b);
-                        if (dd < r && cross2(b - a, x - a) * cross2(cc - b, x - b) < 0.0f) {
+                        if (act && dd < r && cross2(b - a, x - a) * cross2(cc - b, x - b) < 0.0f) {
                             r = dd;
                             found = true;
                             best_p = b;
@@ -848,7 +872,9 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                 }
             }
             if (sp == 0) break;
-            cur = uniform(wst[--sp]);
+            --sp;
+            cur = uniform(wst[sp]);
+            act = ((wmk[sp] >> lane) & 1ull) != 0ull;
         }
     }
     if (!live) return;
@@ -864,6 +890,118 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
     if (o.p) gqd::st(o.p + (size_t)D * qi, best_p);
     if (o.d) o.d[qi] = best_d;
     if (o.prim_id) o.prim_id[qi] = P.ids[best_slot];
+}
+
+// Output record of a closest-silhouette query from the winning silhouette slot (-1: none).  The
+// point and distance are recomputed exactly as the leaf test computed them.
+template <int D>
+__device__ __forceinline__ void sil_record(const EngineParams &P, uint32_t qi, const typename VT<D>::T &x,
+                                           int best_slot) {
+    using V = typename VT<D>::T;
+    const gq_records &o = P.out;
+    if (best_slot < 0) {
+        if (o.valid) o.valid[qi] = 0;
+        if (o.p) for (int j = 0; j < D; ++j) o.p[(size_t)D * qi + j] = 0.0f;
+        if (o.d) o.d[qi] = 0.0f;
+        if (o.prim_id) o.prim_id[qi] = 0;
+        return;
+    }
+    const uint32_t s = (uint32_t)best_slot;
+    V bp;
+    float bd;
+    if constexpr (D == 3) {
+        const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1];
+        bd = closest_point_segment(x, v3(s0.x, s0.y, s0.z), v3(s0.w, s1.x, s1.y), bp);
+    } else {
+        const float4 s0 = P.prims[2 * s];
+        bp = v2(s0.z, s0.w);
+        bd = norm(x - bp);
+    }
+    if (o.valid) o.valid[qi] = 1;
+    if (o.p) gqd::st(o.p + (size_t)D * qi, bp);
+    if (o.d) o.d[qi] = bd;
+    if (o.prim_id) o.prim_id[qi] = P.ids[s];
+}
+
+// ---------------------------------------------------------------------------------------------
+// closest silhouette, lane per query: the same node tests as the packet pass, one query per lane
+// over the Morton order with a per-lane LDS stack (one entry per binary level at most).  Cheaper
+// than the packet pass when per-query work is small and culling is effective (2D silhouettes).
+template <int D, bool Q>
+__global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
+    using V = typename VT<D>::T;
+    GQ_LANE_STACK_DECL
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t qi = query_index<D>(P, i);
+    const V x = ldv<D>(P.x + (size_t)D * qi);
+    float xc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < D; ++k) xc[k] = comp<D>(x, k);
+    float r = P.r_max ? P.r_max[qi] : P.r_max_scalar;
+    int best_slot = -1;
+    float d_root, d_unused;
+    box_distance(ldv<D>(P.root_lo), ldv<D>(P.root_hi), x, d_root, d_unused);
+    bool go = d_root <= r;   // reference: a query that misses the root box is not traversed
+    uint32_t cur = P.root;
+    while (go) {
+        if (!ref_is_leaf(cur)) {
+            PairChild c[2];
+            load_pair<D, Q>(P.nodes, cur, c);
+            bool h[2];
+            float key[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                float dm, dx;
+                box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
+                bool hj = dm <= r;
+                if (hj && c[j].ref_cone >= 0) {
+                    const RefCone &rc = P.ref_cones[c[j].ref_cone];
+                    float dr;
+                    hj = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dr);
+                }
+                if (hj && exact_cull<D>(c[j], xc)) hj = false;
+                h[j] = hj;
+                key[j] = dm;
+            }
+            if (h[0] && h[1]) {
+                const int near = key[1] < key[0] ? 1 : 0;
+                GQ_LPUSH(c[1 - near].child, key[1 - near]);
+                cur = c[near].child;
+                continue;
+            }
+            if (h[0] || h[1]) {
+                cur = h[0] ? c[0].child : c[1].child;
+                continue;
+            }
+        } else {
+            uint32_t first, count;
+            leaf_range(cur, P.big, first, count);
+            for (uint32_t s = first; s < first + count; ++s) {
+                if constexpr (D == 3) {
+                    const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1], s2 = P.prims[3 * s + 2];
+                    V3 pt;
+                    const float dd = closest_point_segment(x, v3(s0.x, s0.y, s0.z), v3(s0.w, s1.x, s1.y), pt);
+                    if (dd < r) {
+                        if (sil_edge_test(pt - x, s1, s2)) {
+                            r = dd;
+                            best_slot = (int)s;
+                        }
+                    }
+                } else {
+                    const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
+                    const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
+                    const float dd = norm(x - b);
+                    if (dd < r && cross2(b - a, x - a) * cross2(cc - b, x - b) < 0.0f) {
+                        r = dd;
+                        best_slot = (int)s;
+                    }
+                }
+            }
+        }
+        GQ_LPOP(r, cur, go);
+    }
+    sil_record<D>(P, qi, x, best_slot);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -936,9 +1074,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
                             V3 pt;
                             const float dd = closest_point_segment(x, a, b, pt);
                             if (dd < r && (dd < cand || (dd == cand && (int)s < cand_slot))) {
-                                const V3 n0 = v3(s1.z, s1.w, s2.x), n1 = v3(s2.y, s2.z, s2.w);
-                                const V3 vd = pt - x;
-                                if (dot(vd, n0) * dot(vd, n1) < 0.0f) {
+                                if (sil_edge_test(pt - x, s1, s2)) {
                                     cand = dd;
                                     cand_slot = (int)s;
                                 }
@@ -980,31 +1116,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
             if (h1 && pos < kFrontSil) front[pos] = ch1;
             size = size + tot < kFrontSil ? size + tot : kFrontSil;
         }
-        {
-            const gq_records &o = P.out;
-            if (best_slot < 0) {
-                if (o.valid) o.valid[qi] = 0;
-                if (o.p) for (int j = 0; j < D; ++j) o.p[(size_t)D * qi + j] = 0.0f;
-                if (o.d) o.d[qi] = 0.0f;
-                if (o.prim_id) o.prim_id[qi] = 0;
-            } else {
-                const uint32_t s = (uint32_t)best_slot;
-                V bp;
-                float bd;
-                if constexpr (D == 3) {
-                    const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1];
-                    bd = closest_point_segment(x, v3(s0.x, s0.y, s0.z), v3(s0.w, s1.x, s1.y), bp);
-                } else {
-                    const float4 s0 = P.prims[2 * s];
-                    bp = v2(s0.z, s0.w);
-                    bd = norm(x - bp);
-                }
-                if (o.valid) o.valid[qi] = 1;
-                if (o.p) gqd::st(o.p + (size_t)D * qi, bp);
-                if (o.d) o.d[qi] = bd;
-                if (o.prim_id) o.prim_id[qi] = P.ids[s];
-            }
-        }
+        sil_record<D>(P, qi, x, best_slot);
     }
 }
 
@@ -1130,6 +1242,10 @@ hipError_t launch_traversal(QueryKind kind, const EngineParams &P, hipStream_t s
             hipLaunchKernelGGL((k_closest_point<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
             if (P.budget > 0)
                 hipLaunchKernelGGL((k_closest_point_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, P);
+        } else if (P.sil_lane) {
+            const unsigned lgrid = (unsigned)((P.n + kRayBlock - 1) / kRayBlock);
+            const size_t lds = (size_t)P.stack_depth * kRayBlock * sizeof(uint2);
+            hipLaunchKernelGGL((k_silhouette_lane<D, Q>), dim3(lgrid), dim3(kRayBlock), lds, st, P);
         } else {
             if (P.budget > 0) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, P.defer_count, 2);
             hipLaunchKernelGGL((k_silhouette<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
@@ -1188,6 +1304,10 @@ hipError_t launch_dim(QueryKind kind, const WideView &w, bool quantized, const Q
     P.big = w.big_leaves;
     P.stack_depth = 3 * w.wide_depth + 4;
     if (kind == kSilhouette) {
+        // lane per query wins where culling leaves little work per query (2D); 3D interior queries
+        // need the packet pass and its wave-per-query tail
+        P.sil_lane = (w.traversal == 2 || (w.traversal == 0 && D == 2)) && w.bin_depth + 2 <= kRayStackMax;
+        if (P.sil_lane) P.stack_depth = w.bin_depth + 2;
         P.nodes = quantized ? w.qsnodes : w.snodes;
         P.prims = w.sils;
         P.ids = w.sil_ids;
@@ -1205,7 +1325,7 @@ hipError_t launch_dim(QueryKind kind, const WideView &w, bool quantized, const Q
     P.budget = 0;
     hipError_t e = sort_queries<D>(q.x, P.n, st, &P.perm, &buf);
     const bool defer = (kind == kClosestPoint && 256 * (w.wide_depth + 1) <= kFront) ||
-                       (kind == kSilhouette && 128 * (w.bin_depth + 2) <= kFrontSil);
+                       (kind == kSilhouette && !P.sil_lane && 128 * (w.bin_depth + 2) <= kFrontSil);
     if (e == hipSuccess && defer && w.packet_budget >= 0) {
         e = hipMallocAsync(&dbuf, 256 + (size_t)P.n * sizeof(uint4), st);
         if (e == hipSuccess) {

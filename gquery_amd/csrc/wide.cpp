@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
 
 namespace gqa {
 namespace {
@@ -203,7 +204,8 @@ inline void fnormalize(float *v) {
 
 struct SilNormals {
     double n0[3], n1[3];
-    bool ok;
+    bool ok;      // the predicate can hold for some query point
+    bool always;  // boundary edge: a silhouette for every query point
 };
 
 template <int D>
@@ -225,28 +227,38 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
             fnormalize(n0);
             fcross(ea, ba, n1);
             fnormalize(n1);
+            // an edge with one adjacent face is a silhouette for every query point
+            // (silhouette_edge.py:54): tagged in the n1.x slot, which the kernels test first
+            const bool boundary = (int)r[12] == -1 || (int)r[15] == -1;
             out.sils[3 * s + 0] = make_float4(a[0], a[1], a[2], bb[0]);
             out.sils[3 * s + 1] = make_float4(bb[1], bb[2], n0[0], n0[1]);
-            out.sils[3 * s + 2] = make_float4(n0[2], n1[0], n1[1], n1[2]);
+            out.sils[3 * s + 2] = make_float4(n0[2], boundary ? u2f(kSilAlwaysTag) : n1[0], n1[1], n1[2]);
             // exact-cull cones bound the float32 normals the predicate actually uses; a non-finite
             // normal makes the predicate NaN, i.e. never a silhouette
             SilNormals &sn = nrm[s];
             sn.ok = true;
+            sn.always = boundary;
             for (int k = 0; k < 3; ++k) {
                 sn.n0[k] = n0[k];
                 sn.n1[k] = n1[k];
                 sn.ok = sn.ok && std::isfinite(n0[k]) && std::isfinite(n1[k]);
             }
+            sn.ok = sn.ok || boundary;
             out.sil_ids.push_back((int32_t)r[18]);
         }
     } else {
         out.sils.resize(ns * 2);
         for (size_t s = 0; s < ns; ++s) {
             const float *r = h.silhouette_data.data() + s * ss;
-            out.sils[2 * s + 0] = make_float4(r[0], r[1], r[2], r[3]);
+            // a vertex missing a neighbour is never a silhouette (silhouette_vertex.py:46): its
+            // first point becomes NaN, so the kernels' cross-product predicate is false
+            const bool open_end = (int)r[6] == -1 || (int)r[8] == -1;
+            const float qn = std::numeric_limits<float>::quiet_NaN();
+            out.sils[2 * s + 0] = make_float4(open_end ? qn : r[0], open_end ? qn : r[1], r[2], r[3]);
             out.sils[2 * s + 1] = make_float4(r[4], r[5], 0.0f, 0.0f);
             out.sil_ids.push_back((int32_t)r[9]);
-            nrm[s].ok = true;
+            nrm[s].ok = !open_end;
+            nrm[s].always = false;
         }
     }
     // subtree silhouette ranges (contiguous in leaf order)
@@ -264,9 +276,13 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
     auto cone_of = [&](size_t i, ConeF &cf) {
         cf.axis[0] = 0; cf.axis[1] = 0; cf.axis[2] = 1;
         cf.cos_t = 0.0f; cf.sin_t = 1.0f;  // never culls
-        bool any = false;
-        for (uint32_t s = sb[i]; s < sb[i] + sc[i] && !any; ++s) any = nrm[s].ok;
+        bool any = false, always = false;
+        for (uint32_t s = sb[i]; s < sb[i] + sc[i]; ++s) {
+            any = any || nrm[s].ok;
+            always = always || nrm[s].always;
+        }
         if (!any) { cf.cos_t = 2.0f; cf.sin_t = 0.0f; return; }  // no possible silhouette below: always cull
+        if (always) return;                                        // a boundary edge below: never cull
         if (D != 3) return;
         // orient each (n0, n1) pair along a common direction (the predicate is invariant under
         // flipping both), then bound all of them by one cone

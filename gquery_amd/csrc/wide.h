@@ -19,6 +19,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <vector>
 
 #include "builder.h"
@@ -27,6 +28,14 @@
 namespace gqa {
 
 constexpr uint32_t kLeafBit = 0x80000000u;
+// Quiet-NaN bit pattern in the n1.x slot of a 3D silhouette record: the edge has one adjacent face
+// and is a silhouette for every query point.
+constexpr uint32_t kSilAlwaysTag = 0x7FC5B0EDu;
+inline float u2f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
 constexpr int kMaxLeafPrims = 127;
 __host__ __device__ inline bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0; }
 __host__ __device__ inline uint32_t ref_count(uint32_t r) { return (r >> 24) & 0x7Fu; }

@@ -95,7 +95,9 @@ typedef struct gq_scene_options {
     int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
                               move to the wave-per-query pass; 0 = default (closest point 1500 in
                               3D / 1000 in 2D, silhouettes 1000), < 0 = never */
-    int32_t reserved[6];
+    int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D, packet + wave
+                              pass in 3D), 1 = packet + wave pass, 2 = lane per query */
+    int32_t reserved[5];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
 /* info[0..7] = dim, has_snch, device bytes, n_nodes, n_wide_nodes, n_prims, n_silhouettes, 0 */

@@ -685,7 +685,9 @@ void or_bf_intersect_f64(int D, const float *V, const int32_t *F, int64_t nf, co
 }
 
 /* Nearest valid silhouette by brute force in double over the raw silhouette records (one per
- * vertex in 2D, per edge in 3D; flat layout of silhouette_vertex.h / silhouette_edge.h). */
+ * vertex in 2D, per edge in 3D; flat layout of silhouette_vertex.h / silhouette_edge.h).  A 2D
+ * vertex missing a neighbour is never a silhouette (silhouette_vertex.py:46); a 3D edge missing a
+ * face always is (silhouette_edge.py:54). */
 void or_bf_silhouette_f64(int D, const float *raw, int64_t ns, const float *x, int64_t n,
                           double *best_d, double *best_p, int32_t *best_id, double *second_d) {
     const int SS = D == 2 ? 10 : 19;
@@ -702,7 +704,7 @@ void or_bf_silhouette_f64(int D, const float *raw, int64_t ns, const float *x, i
                 double a[2] = {sl[0], sl[1]}, b[2] = {sl[2], sl[3]}, c[2] = {sl[4], sl[5]};
                 double c0 = (b[0] - a[0]) * (p[1] - a[1]) - (b[1] - a[1]) * (p[0] - a[0]);
                 double c1 = (c[0] - b[0]) * (p[1] - b[1]) - (c[1] - b[1]) * (p[0] - b[0]);
-                if (c0 * c1 < 0.0) {
+                if ((int)sl[6] != -1 && (int)sl[8] != -1 && c0 * c1 < 0.0) {
                     dd = sqrt((p[0] - b[0]) * (p[0] - b[0]) + (p[1] - b[1]) * (p[1] - b[1]));
                     pt[0] = b[0]; pt[1] = b[1];
                 }
@@ -714,7 +716,7 @@ void or_bf_silhouette_f64(int D, const float *raw, int64_t ns, const float *x, i
                 for (int k = 0; k < 3; ++k) { ba[k] = b[k] - a[k]; ca[k] = c[k] - a[k]; ea[k] = e[k] - a[k]; vd[k] = pt[k] - p[k]; }
                 n0[0] = ba[1] * ca[2] - ba[2] * ca[1]; n0[1] = ba[2] * ca[0] - ba[0] * ca[2]; n0[2] = ba[0] * ca[1] - ba[1] * ca[0];
                 n1[0] = ea[1] * ba[2] - ea[2] * ba[1]; n1[1] = ea[2] * ba[0] - ea[0] * ba[2]; n1[2] = ea[0] * ba[1] - ea[1] * ba[0];
-                if (dotd(3, vd, n0) * dotd(3, vd, n1) < 0.0) dd = dseg;
+                if ((int)sl[12] == -1 || (int)sl[15] == -1 || dotd(3, vd, n0) * dotd(3, vd, n1) < 0.0) dd = dseg;
             }
             if (dd < b1) { b2 = b1; b1 = dd; id = (int32_t)(int)sl[SS - 1]; memcpy(bp, pt, sizeof bp); }
             else if (dd < b2) b2 = dd;
