@@ -111,6 +111,69 @@ def pmc_summary(args):
     print(json.dumps(res, indent=1))
 
 
+def accuracy(args):
+    """Per query type and method: errors against the FP64 brute force (oracle, double precision over
+    every primitive) and the FP32 restatement of the reference traversal, on N queries sampled from
+    the headline distribution.  Prints one JSON object per (scene, query, method)."""
+    import numpy as np
+    import torch
+    from gquery_amd import synth
+    from gquery_amd.shapes.mesh import Mesh
+    from gquery_amd.shapes.polyline import Polyline
+    from oracle import oracle as O
+    n = args.accuracy
+    scenes = [("mesh s=%d" % args.subdiv, 3, *synth.displaced_icosphere(args.subdiv)),
+              ("polyline %d" % args.segments, 2, *synth.star_polyline(args.segments))]
+    for name, dim, V, F in scenes:
+        shape = (Mesh if dim == 3 else Polyline)(V, F, device="cuda:0")
+        s = shape.snch
+        x = synth.uniform_points(n, dim, seed=101)
+        v = synth.uniform_directions(n, dim, seed=102)
+        for kind in ("cp", "ray", "sil"):
+            t0 = time.perf_counter()
+            if kind == "cp":
+                ref = O.closest_point(dim, s.bvh_node_data(), s.primitive_data(), x)
+                bf = O.bf_closest_point(dim, V, F, x)
+                bd, b2 = bf["d"], bf["second_d"]
+            elif kind == "ray":
+                ref = O.intersect(dim, s.bvh_node_data(), s.primitive_data(), x, v)
+                bf = O.bf_intersect(dim, V, F, x, v)
+                bd, b2 = bf["t"], bf["second_t"]
+            else:
+                ref = O.silhouette(dim, s.node_data(), s.silhouette_data(), x)
+                bf = O.bf_silhouette(dim, s.raw_silhouette_data(), x)
+                bd, b2 = bf["d"], bf["second_d"]
+            bvalid = np.isfinite(bd)
+            with np.errstate(invalid="ignore"):
+                gap = bvalid & (~np.isfinite(b2) | (b2 - bd > 1e-5 * np.maximum(bd, 1e-30)))
+            log(f"{name} {kind}: oracle + FP64 brute force {time.perf_counter() - t0:.1f}s")
+            for method in ("plain", "engine_fp32", "engine"):
+                if kind == "cp":
+                    r = shape.closest_point_bvh(x, method=method)
+                elif kind == "ray":
+                    r = shape.intersect_bvh(x, v, method=method)
+                else:
+                    r = shape.closest_silhouette_snch(x, method=method)
+                torch.cuda.synchronize()
+                gv = r.valid.cpu().numpy().astype(bool)
+                gd = r.d.cpu().numpy().astype(np.float64)
+                gid = r.prim_id.cpu().numpy()
+                both = gv & bvalid
+                same = both & gap & (gid == bf["prim_id"])
+                # error relative to max(d, 1): relative beyond the scene scale (radius ~1), absolute
+                # below it, where float32 coordinates carry absolute (not relative) error
+                rel64 = np.abs(gd - bd) / np.maximum(np.abs(np.nan_to_num(bd)), 1.0)
+                rel32 = np.abs(gd - ref["d"]) / np.maximum(np.abs(ref["d"]), 1.0)
+                out = {"scene": name, "query": kind, "method": method, "n": n,
+                       "valid_vs_fp64": float((gv == bvalid).mean()),
+                       "id_vs_fp64_where_gap": float(same.sum() / max((both & gap).sum(), 1)),
+                       "max_err_d_vs_fp64": float(rel64[same].max()) if same.any() else 0.0,
+                       "p99_err_d_vs_fp64": float(np.percentile(rel64[same], 99)) if same.any() else 0.0,
+                       "max_err_d_vs_fp32_restatement": float(rel32[gv & ref["valid"].astype(bool)].max())
+                       if (gv & ref["valid"].astype(bool)).any() else 0.0}
+                print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +198,13 @@ def main():
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="no GPU: turn two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; csv) of "
                          "`bench.py --copy-mb M` into profiles/traffic_<workload>_<method>.{json,md}")
+    ap.add_argument("--accuracy", type=int, default=0, metavar="N",
+                    help="instead of timing: error of every method and query type against the FP64 "
+                         "brute force and the FP32 restatement on N sampled queries")
     args = ap.parse_args()
+    if args.accuracy:
+        accuracy(args)
+        return
     if args.pmc_summary:
         pmc_summary(args)
         return

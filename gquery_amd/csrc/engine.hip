@@ -20,6 +20,7 @@
 
 #include "gq_device.h"
 #include "gq_internal.h"
+#include "robust.h"
 #include "wide.h"
 
 namespace gqa {
@@ -269,12 +270,13 @@ __device__ __forceinline__ void cp_record(const EngineParams &P, uint32_t qi, co
     if (o.valid) o.valid[qi] = 1;
 }
 
-// Silhouette-edge predicate on an engine record (s1, s2 hold n0 and n1): a boundary edge is a
-// silhouette everywhere (silhouette_edge.py:54), else is_silhouette_edge (silhouette_edge.py:7-12) on the view
-// vector from the query to the nearest point of the edge.
-__device__ __forceinline__ bool sil_edge_test(const V3 &vd, const float4 &s1, const float4 &s2) {
+// Silhouette-edge predicate on an engine record (s1, s2 hold n0 and n1; a, b the edge, pt its
+// nearest point to x): a boundary edge is a silhouette everywhere (silhouette_edge.py:54), else
+// is_silhouette_edge (silhouette_edge.py:7-12) with the FP64 fallback near the sign change.
+__device__ __forceinline__ bool sil_edge_test(const V3 &x, const V3 &a, const V3 &b, const V3 &pt,
+                                              const float4 &s1, const float4 &s2) {
     if (__float_as_uint(s2.y) == kSilAlwaysTag) return true;
-    return dot(vd, v3(s1.z, s1.w, s2.x)) * dot(vd, v3(s2.y, s2.z, s2.w)) < 0.0f;
+    return sil_edge_pred_robust(x, a, b, pt, v3(s1.z, s1.w, s2.x), v3(s2.y, s2.z, s2.w));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -381,7 +383,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
                     V2 pt;
                     dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
                 }
-                if (live && dd < best) {
+                if (live && (dd < best || (dd == best && (int)s < best_slot))) {
                     best = dd;
                     best_slot = (int)s;
                     bound2 = dd * dd * 1.000002f;
@@ -501,7 +503,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
             }
             cand = uniformf(cand);
             cand_slot = (int)uniform((uint32_t)cand_slot);
-            if (cand < best) {
+            if (cand < best || (cand == best && cand_slot < best_slot)) {
                 best = cand;
                 best_slot = cand_slot;
                 bound2 = cand * cand * 1.000002f;
@@ -585,12 +587,13 @@ __global__ void __launch_bounds__(kRayBlock) k_intersect(EngineParams P) {
                 float t;
                 if constexpr (D == 3) {
                     const float4 a = P.prims[4 * s], b = P.prims[4 * s + 1], c = P.prims[4 * s + 2];
-                    t = ray_triangle_t(x, v, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z));
+                    float bv, bw;
+                    t = ray_triangle_robust(x, v, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), bv, bw);
                 } else {
                     const float4 ab = P.prims[s];
                     t = ray_segment_t(x, v, v2(ab.x, ab.y), v2(ab.z, ab.w));
                 }
-                if (t < best) {
+                if (t < best || (t == best && (int)s < best_slot)) {
                     best = t;
                     best_slot = (int)s;
                 }
@@ -607,23 +610,9 @@ __global__ void __launch_bounds__(kRayBlock) k_intersect(EngineParams P) {
             const float4 a4 = P.prims[4 * s], b4 = P.prims[4 * s + 1], c4 = P.prims[4 * s + 2];
             const V3 a = v3(a4.x, a4.y, a4.z), b = v3(b4.x, b4.y, b4.z), c = v3(c4.x, c4.y, c4.z);
             const V3 v1 = b - a, v2_ = c - a;
-            const V3 pp = cross(v, v2_);
-            const float det = dot(v1, pp);
-            float vv = 0.0f, ww = 0.0f, t = 0.0f;
-            ok = false;
-            if (fabsf(det) > kDrEps) {
-                const float inv_det = 1.0f / det;
-                const V3 sv = x - a;
-                vv = dot(sv, pp) * inv_det;
-                if ((vv >= 0.0f) & (vv <= 1.0f)) {
-                    const V3 qq = cross(sv, v1);
-                    ww = dot(v, qq) * inv_det;
-                    if ((ww >= 0.0f) & (vv + ww <= 1.0f)) {
-                        t = dot(v2_, qq) * inv_det;
-                        ok = (t >= 0.0f) & (t <= best);
-                    }
-                }
-            }
+            float vv, ww;
+            const float t = ray_triangle_robust(x, v, a, b, c, vv, ww);   // as in the traversal
+            ok = t <= best;
             if (ok) {
                 if (o.p) gqd::st(o.p + 3 * (size_t)qi, a + v1 * vv + v2_ * ww);
                 if (o.n) gqd::st(o.n + 3 * (size_t)qi, normalize(cross(v1, v2_)));
@@ -848,8 +837,8 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                         const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
                         V3 pt;
                         const float dd = closest_point_segment(x, a, b, pt);
-                        if (act && dd < r) {
-                            if (sil_edge_test(pt - x, s1, s2)) {
+                        if (act && (dd < r || (dd == r && found && (int)s < best_slot))) {
+                            if (sil_edge_test(x, a, b, pt, s1, s2)) {
                                 r = dd;
                                 found = true;
                                 best_p = pt;
@@ -861,7 +850,8 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                         const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
                         const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
                         const float dd = norm(x - b);
-                        if (act && dd < r && cross2(b - a, x - a) * cross2(cc - b, x - b) < 0.0f) {
+                        if (act && (dd < r || (dd == r && found && (int)s < best_slot)) &&
+                            sil_vertex_pred_robust(x, a, b, cc)) {
                             r = dd;
                             found = true;
                             best_p = b;
@@ -980,10 +970,11 @@ __global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
             for (uint32_t s = first; s < first + count; ++s) {
                 if constexpr (D == 3) {
                     const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1], s2 = P.prims[3 * s + 2];
+                    const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
                     V3 pt;
-                    const float dd = closest_point_segment(x, v3(s0.x, s0.y, s0.z), v3(s0.w, s1.x, s1.y), pt);
-                    if (dd < r) {
-                        if (sil_edge_test(pt - x, s1, s2)) {
+                    const float dd = closest_point_segment(x, a, b, pt);
+                    if (dd < r || (dd == r && best_slot >= 0 && (int)s < best_slot)) {
+                        if (sil_edge_test(x, a, b, pt, s1, s2)) {
                             r = dd;
                             best_slot = (int)s;
                         }
@@ -992,7 +983,8 @@ __global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
                     const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
                     const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
                     const float dd = norm(x - b);
-                    if (dd < r && cross2(b - a, x - a) * cross2(cc - b, x - b) < 0.0f) {
+                    if ((dd < r || (dd == r && best_slot >= 0 && (int)s < best_slot)) &&
+                        sil_vertex_pred_robust(x, a, b, cc)) {
                         r = dd;
                         best_slot = (int)s;
                     }
@@ -1073,8 +1065,9 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
                             const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
                             V3 pt;
                             const float dd = closest_point_segment(x, a, b, pt);
-                            if (dd < r && (dd < cand || (dd == cand && (int)s < cand_slot))) {
-                                if (sil_edge_test(pt - x, s1, s2)) {
+                            if ((dd < r || (dd == r && best_slot >= 0 && (int)s < best_slot)) &&
+                                (dd < cand || (dd == cand && (int)s < cand_slot))) {
+                                if (sil_edge_test(x, a, b, pt, s1, s2)) {
                                     cand = dd;
                                     cand_slot = (int)s;
                                 }
@@ -1083,8 +1076,9 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
                             const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
                             const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
                             const float dd = norm(x - b);
-                            if (dd < r && (dd < cand || (dd == cand && (int)s < cand_slot)) &&
-                                cross2(b - a, x - a) * cross2(cc - b, x - b) < 0.0f) {
+                            if ((dd < r || (dd == r && best_slot >= 0 && (int)s < best_slot)) &&
+                                (dd < cand || (dd == cand && (int)s < cand_slot)) &&
+                                sil_vertex_pred_robust(x, a, b, cc)) {
                                 cand = dd;
                                 cand_slot = (int)s;
                             }
@@ -1103,7 +1097,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
             }
             cand = uniformf(cand);
             cand_slot = (int)uniform((uint32_t)cand_slot);
-            if (cand < r) {
+            if (cand < r || (cand == r && best_slot >= 0 && cand_slot >= 0 && cand_slot < best_slot)) {
                 r = cand;
                 best_slot = cand_slot;
             }
