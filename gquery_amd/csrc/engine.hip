@@ -17,6 +17,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "gq_device.h"
 #include "gq_internal.h"
@@ -48,15 +50,105 @@ struct EngineParams {
     float root_lo[3], root_hi[3];
     const float *x, *v, *r_max;
     float r_max_scalar;
-    const uint32_t *perm;
+    uint32_t *perm;           // Morton-cell order of the queries (null: input order)
+    uint32_t *cell, *counts, *part, *bounds;   // counting-sort scratch (per job)
+    int cell_bits;            // Morton cell bits per axis
+    int n_cells;              // 2^(cell_bits * dimension)
+    int cell_shift;           // cell = fine Morton code >> cell_shift
+    uint32_t job;             // index in the job table (carried by deferred entries)
     int n;
     int stack_depth;          // lane-stack entries (ray kernel)
     int budget;               // packet steps before a wave hands its queries to the wave-per-query pass (0 = never)
     uint32_t *defer_count;    // [0] deferred queries, [1] work counter of the wave-per-query pass
-    uint4 *deferred;          // (query index, best distance bits, best slot, 0)
+    uint4 *deferred;          // (query index, best distance bits, best slot, job)
     int sil_lane;             // silhouettes: lane-per-query kernel instead of packet + wave pass
     gq_records out;
 };
+
+// Device view of a job: the same fields with every pointer in the global address space.  Pointers
+// read from a job table are generic to the compiler, which would access them with flat
+// instructions; the explicit cast of each one makes every access a global load or store.
+#define GQ_G __attribute__((address_space(1)))
+struct GRecords {
+    GQ_G uint8_t *valid;
+    GQ_G float *p, *n, *uv, *d;
+    GQ_G int32_t *prim_id;
+    GQ_G uint8_t *on_boundary;
+};
+struct GP {
+    const GQ_G uint8_t *nodes;
+    const GQ_G float4 *prims;
+    const GQ_G int32_t *ids;
+    const GQ_G RefCone *ref_cones;
+    const GQ_G uint32_t *big;
+    uint32_t root;
+    float root_lo[3], root_hi[3];
+    const GQ_G float *x, *v, *r_max;
+    float r_max_scalar;
+    const GQ_G uint32_t *perm;
+    uint32_t job;
+    int n;
+    int stack_depth;
+    int budget;
+    GQ_G uint32_t *defer_count;
+    GQ_G uint4 *deferred;
+    GRecords out;
+};
+template <typename T> __device__ __forceinline__ GQ_G T *gcast(T *p) { return (GQ_G T *)p; }
+typedef float gq_v4f __attribute__((ext_vector_type(4)));
+typedef unsigned int gq_v4u __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldg(const GQ_G float4 *p) {
+    const gq_v4f v = *(const GQ_G gq_v4f *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg(GQ_G uint4 *p, uint4 v) { *(GQ_G gq_v4u *)p = gq_v4u{v.x, v.y, v.z, v.w}; }
+// Global-pointer forms of the generic helpers (the cast back to a generic pointer keeps a global
+// source, so the accesses stay global instructions after inlining).
+template <int D> __device__ __forceinline__ typename VT<D>::T ldv(const GQ_G float *p) { return gqd::ldv<D>((const float *)p); }
+template <int D> __device__ __forceinline__ typename VT<D>::T ldv(const float *p) { return gqd::ldv<D>(p); }
+__device__ __forceinline__ void st(GQ_G float *p, const V3 &v) { gqd::st((float *)p, v); }
+__device__ __forceinline__ void st(GQ_G float *p, const V2 &v) { gqd::st((float *)p, v); }
+
+__device__ __forceinline__ RefCone ldg(const GQ_G RefCone *p) {
+    const GQ_G float *f = (const GQ_G float *)p;
+    RefCone r;
+    r.axis[0] = f[0]; r.axis[1] = f[1]; r.axis[2] = f[2];
+    r.half = f[3]; r.radius = f[4];
+    r.lo[0] = f[5]; r.lo[1] = f[6]; r.lo[2] = f[7];
+    r.hi[0] = f[8]; r.hi[1] = f[9]; r.hi[2] = f[10];
+    return r;
+}
+
+
+__device__ __forceinline__ GP gview(const EngineParams &in) {
+    GP P;
+    P.nodes = gcast((const uint8_t *)in.nodes);
+    P.prims = gcast(in.prims);
+    P.ids = gcast(in.ids);
+    P.ref_cones = gcast(in.ref_cones);
+    P.big = gcast(in.big);
+    P.root = in.root;
+    for (int k = 0; k < 3; ++k) { P.root_lo[k] = in.root_lo[k]; P.root_hi[k] = in.root_hi[k]; }
+    P.x = gcast(in.x);
+    P.v = gcast(in.v);
+    P.r_max = gcast(in.r_max);
+    P.r_max_scalar = in.r_max_scalar;
+    P.perm = gcast((const uint32_t *)in.perm);
+    P.job = in.job;
+    P.n = in.n;
+    P.stack_depth = in.stack_depth;
+    P.budget = in.budget;
+    P.defer_count = gcast(in.defer_count);
+    P.deferred = gcast(in.deferred);
+    P.out.valid = gcast(in.out.valid);
+    P.out.p = gcast(in.out.p);
+    P.out.n = gcast(in.out.n);
+    P.out.uv = gcast(in.out.uv);
+    P.out.d = gcast(in.out.d);
+    P.out.prim_id = gcast(in.out.prim_id);
+    P.out.on_boundary = gcast(in.out.on_boundary);
+    return P;
+}
 
 // ---- lane-per-query stack (rays): LDS only, depth chosen per launch from the tree depth, one
 // column per lane of a [depth][blockDim] dynamic shared array.  No private spill area, so the
@@ -223,15 +315,15 @@ template <int D> __device__ __forceinline__ float comp(const typename VT<D>::T &
     else return k == 0 ? p.x : (k == 1 ? p.y : p.z);
 }
 
-template <int D> __device__ __forceinline__ uint32_t query_index(const EngineParams &P, int i) {
+template <int D> __device__ __forceinline__ uint32_t query_index(const GP &P, int i) {
     return P.perm ? P.perm[i] : (uint32_t)i;
 }
 
 // Output record of a closest-point query from the winning primitive slot (reference formulas).
 template <int D>
-__device__ __forceinline__ void cp_record(const EngineParams &P, uint32_t qi, const typename VT<D>::T &p, int best_slot) {
+__device__ __forceinline__ void cp_record(const GP &P, uint32_t qi, const typename VT<D>::T &p, int best_slot) {
     using V = typename VT<D>::T;
-    const gq_records &o = P.out;
+    const GRecords &o = P.out;
     if (best_slot < 0) {
         if (o.valid) o.valid[qi] = 0;
         if (o.p) for (int k = 0; k < D; ++k) o.p[(size_t)D * qi + k] = 0.0f;
@@ -244,25 +336,25 @@ __device__ __forceinline__ void cp_record(const EngineParams &P, uint32_t qi, co
     const int s = best_slot;
     if constexpr (D == 3) {
         // Triangle.closest_point (triangle.py:75-81)
-        const float4 a4 = P.prims[4 * s], b4 = P.prims[4 * s + 1], c4 = P.prims[4 * s + 2];
+        const float4 a4 = ldg(P.prims + (4 * s)), b4 = ldg(P.prims + (4 * s + 1)), c4 = ldg(P.prims + (4 * s + 2));
         const V3 a = v3(a4.x, a4.y, a4.z), b = v3(b4.x, b4.y, b4.z), c = v3(c4.x, c4.y, c4.z);
         V3 pt;
         V2 uv;
         const float dd = closest_point_triangle(p, a, b, c, pt, uv);
-        if (o.p) gqd::st(o.p + 3 * (size_t)qi, pt);
-        if (o.n) gqd::st(o.n + 3 * (size_t)qi, normalize(cross(b - a, c - a)));
-        if (o.uv) gqd::st(o.uv + 2 * (size_t)qi, uv);
+        if (o.p) st(o.p + 3 * (size_t)qi, pt);
+        if (o.n) st(o.n + 3 * (size_t)qi, normalize(cross(b - a, c - a)));
+        if (o.uv) st(o.uv + 2 * (size_t)qi, uv);
         if (o.d) o.d[qi] = dd;
     } else {
         // LineSegment.closest_point (line_segment.py:63-78)
-        const float4 ab = P.prims[s];
+        const float4 ab = ldg(P.prims + (s));
         const V2 a = v2(ab.x, ab.y), b = v2(ab.z, ab.w);
         const V2 pa = p - a, ba = b - a;
         const V2 t = normalize(ba);
         const float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
         const V2 ip = a + ba * h;
-        if (o.p) gqd::st(o.p + 2 * (size_t)qi, ip);
-        if (o.n) gqd::st(o.n + 2 * (size_t)qi, v2(t.y, -t.x));
+        if (o.p) st(o.p + 2 * (size_t)qi, ip);
+        if (o.n) st(o.n + 2 * (size_t)qi, v2(t.y, -t.x));
         if (o.uv) o.uv[qi] = h;
         if (o.d) o.d[qi] = norm(ip - p);
     }
@@ -291,24 +383,24 @@ template <int D> __device__ __forceinline__ void packet_centroid(bool live, cons
 
 // Hands the live lanes of a wavefront that exceeded its step budget to the wave-per-query pass:
 // query index, current bound and best slot so far, appended with one atomic per wavefront.
-__device__ __forceinline__ void defer_wave(const EngineParams &P, bool live, uint32_t qi, float bound, int slot) {
+__device__ __forceinline__ void defer_wave(const GP &P, bool live, uint32_t qi, float bound, int slot) {
     const unsigned long long m = __ballot(live);
     const uint32_t nlive = (uint32_t)__popcll(m);
     // every lane takes part in the atomic (lane 0 adds the count) so no lane-dependent branch
     // precedes the return
-    const uint32_t base = uniform(atomicAdd(P.defer_count, (threadIdx.x & 63) == 0 ? nlive : 0u));
+    const uint32_t base = uniform(atomicAdd((uint32_t *)P.defer_count, (threadIdx.x & 63) == 0 ? nlive : 0u));
     if (live) {
         const uint32_t rank = (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
-        P.deferred[base + rank] = make_uint4(qi, __float_as_uint(bound), (uint32_t)slot, 0u);
+        stg(P.deferred + base + rank, make_uint4(qi, __float_as_uint(bound), (uint32_t)slot, P.job));
     }
 }
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
+__device__ __forceinline__ void cp_packet(const GP &P, int base) {
     using V = typename VT<D>::T;
     __shared__ uint32_t wstack[kBlock / 64][kPacketDepth];
     uint32_t *wst = wstack[threadIdx.x >> 6];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = base + threadIdx.x;
     const bool live = i < P.n;
     const uint32_t qi = live ? query_index<D>(P, i) : 0u;
     V p;
@@ -333,7 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
         }
         if (!ref_is_leaf(cur)) {
             Node4<D, Q> nd;
-            load_node4<D, Q>(P.nodes, cur, nd);
+            load_node4<D, Q>((const void *)P.nodes, cur, nd);
             float key[4];
             uint32_t ref[4];
 #pragma unroll
@@ -363,14 +455,14 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
             }
         } else {
             uint32_t first, count;
-            leaf_range(cur, P.big, first, count);
+            leaf_range(cur, (const uint32_t *)P.big, first, count);
             first = uniform(first);
             count = uniform(count);
             for (uint32_t s = first; s < first + count; ++s) {
                 float dd;
                 if constexpr (D == 3) {
-                    const float4 a = P.prims[4 * s], b = P.prims[4 * s + 1], c = P.prims[4 * s + 2];
-                    const float4 pl = P.prims[4 * s + 3];
+                    const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
+                    const float4 pl = ldg(P.prims + (4 * s + 3));
                     // plane-distance lower bound, widened for rounding: skip the exact test when no
                     // lane can get strictly closer through this triangle
                     const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
@@ -379,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_point(EngineParams P) {
                     V2 uv;
                     dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
                 } else {
-                    const float4 ab = P.prims[s];
+                    const float4 ab = ldg(P.prims + (s));
                     V2 pt;
                     dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
                 }
@@ -419,19 +511,10 @@ __device__ __forceinline__ int wave_exclusive_scan(int v, int &total) {
 }
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams P) {
+__device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uint2 *front) {
     using V = typename VT<D>::T;
-    __shared__ uint2 front_all[kWaveBlock / 64][kFront];
-    uint2 *front = front_all[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
-    const uint32_t total = uniform(P.defer_count[0]);
-    for (;;) {
-        // All lanes take part in the fetch (lane 0 adds 1, the others 0) and later all lanes store
-        // the same record: no lane-dependent branch inside this loop, so the compiler cannot split
-        // the wavefront across iterations.
-        const uint32_t k = uniform(atomicAdd(P.defer_count + 1, lane == 0 ? 1u : 0u));
-        if (k >= total) break;
-        const uint4 item = P.deferred[k];
+    {
         const uint32_t qi = uniform(item.x);
         float best = uniformf(__uint_as_float(item.y));
         int best_slot = (int)uniform(item.z);
@@ -454,7 +537,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
             if (lane < take && __uint_as_float(e.y) <= bound2) {
                 if (!ref_is_leaf(e.x)) {
                     Node4<D, Q> nd;
-                    load_node4<D, Q>(P.nodes, e.x, nd);
+                    load_node4<D, Q>((const void *)P.nodes, e.x, nd);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         float d2 = 0.0f;
@@ -468,19 +551,19 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
                     }
                 } else {
                     uint32_t first, count;
-                    leaf_range(e.x, P.big, first, count);
+                    leaf_range(e.x, (const uint32_t *)P.big, first, count);
                     for (uint32_t s = first; s < first + count; ++s) {
                         float dd;
                         if constexpr (D == 3) {
-                            const float4 pl = P.prims[4 * s + 3];
+                            const float4 pl = ldg(P.prims + (4 * s + 3));
                             const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
                             if (lb * 0.99999f - 1e-6f >= best) continue;
-                            const float4 a = P.prims[4 * s], b = P.prims[4 * s + 1], c = P.prims[4 * s + 2];
+                            const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
                             V3 pt;
                             V2 uv;
                             dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
                         } else {
-                            const float4 ab = P.prims[s];
+                            const float4 ab = ldg(P.prims + (s));
                             V2 pt;
                             dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
                         }
@@ -527,10 +610,10 @@ __global__ void __launch_bounds__(kWaveBlock) k_closest_point_wave(EngineParams 
 // ---------------------------------------------------------------------------------------------
 // ray (lane per query)
 template <int D, bool Q>
-__global__ void __launch_bounds__(kRayBlock) k_intersect(EngineParams P) {
+__device__ __forceinline__ void ray_lane(const GP &P, int base) {
     using V = typename VT<D>::T;
     GQ_LANE_STACK_DECL
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = base + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t qi = query_index<D>(P, i);
     const V x = ldv<D>(P.x + (size_t)D * qi);
@@ -549,7 +632,7 @@ __global__ void __launch_bounds__(kRayBlock) k_intersect(EngineParams P) {
     while (go) {
         if (!ref_is_leaf(cur)) {
             Node4<D, Q> nd;
-            load_node4<D, Q>(P.nodes, cur, nd);
+            load_node4<D, Q>((const void *)P.nodes, cur, nd);
             float key[4];
             uint32_t ref[4];
             const float lim = best * kHi;
@@ -582,15 +665,15 @@ __global__ void __launch_bounds__(kRayBlock) k_intersect(EngineParams P) {
             cur = ref[0];
         } else {
             uint32_t first, count;
-            leaf_range(cur, P.big, first, count);
+            leaf_range(cur, (const uint32_t *)P.big, first, count);
             for (uint32_t s = first; s < first + count; ++s) {
                 float t;
                 if constexpr (D == 3) {
-                    const float4 a = P.prims[4 * s], b = P.prims[4 * s + 1], c = P.prims[4 * s + 2];
+                    const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
                     float bv, bw;
                     t = ray_triangle_robust(x, v, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), bv, bw);
                 } else {
-                    const float4 ab = P.prims[s];
+                    const float4 ab = ldg(P.prims + (s));
                     t = ray_segment_t(x, v, v2(ab.x, ab.y), v2(ab.z, ab.w));
                 }
                 if (t < best || (t == best && (int)s < best_slot)) {
@@ -601,30 +684,30 @@ __global__ void __launch_bounds__(kRayBlock) k_intersect(EngineParams P) {
             GQ_LPOP(best * kHi, cur, go);
         }
     }
-    const gq_records &o = P.out;
+    const GRecords &o = P.out;
     bool ok = best_slot >= 0;
     const int s = best_slot;
     if (ok) {
         if constexpr (D == 3) {
             // Triangle.ray_intersect (triangle.py:48-73) with r_max = the hit distance
-            const float4 a4 = P.prims[4 * s], b4 = P.prims[4 * s + 1], c4 = P.prims[4 * s + 2];
+            const float4 a4 = ldg(P.prims + (4 * s)), b4 = ldg(P.prims + (4 * s + 1)), c4 = ldg(P.prims + (4 * s + 2));
             const V3 a = v3(a4.x, a4.y, a4.z), b = v3(b4.x, b4.y, b4.z), c = v3(c4.x, c4.y, c4.z);
             const V3 v1 = b - a, v2_ = c - a;
             float vv, ww;
             const float t = ray_triangle_robust(x, v, a, b, c, vv, ww);   // as in the traversal
             ok = t <= best;
             if (ok) {
-                if (o.p) gqd::st(o.p + 3 * (size_t)qi, a + v1 * vv + v2_ * ww);
-                if (o.n) gqd::st(o.n + 3 * (size_t)qi, normalize(cross(v1, v2_)));
-                if (o.uv) gqd::st(o.uv + 2 * (size_t)qi, v2(1.0f - vv - ww, vv));
+                if (o.p) st(o.p + 3 * (size_t)qi, a + v1 * vv + v2_ * ww);
+                if (o.n) st(o.n + 3 * (size_t)qi, normalize(cross(v1, v2_)));
+                if (o.uv) st(o.uv + 2 * (size_t)qi, v2(1.0f - vv - ww, vv));
                 if (o.d) o.d[qi] = t;
             }
         } else {
             // record of bvh.py:267-274
-            const float4 ab = P.prims[s];
+            const float4 ab = ldg(P.prims + (s));
             const V2 tt = v2(ab.z - ab.x, ab.w - ab.y);
-            if (o.p) gqd::st(o.p + 2 * (size_t)qi, x + v * best);
-            if (o.n) gqd::st(o.n + 2 * (size_t)qi, normalize(v2(tt.y, -tt.x)));
+            if (o.p) st(o.p + 2 * (size_t)qi, x + v * best);
+            if (o.n) st(o.n + 2 * (size_t)qi, normalize(v2(tt.y, -tt.x)));
             if (o.uv) o.uv[qi] = -1.0f;
             if (o.d) o.d[qi] = best;
         }
@@ -742,14 +825,14 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
 
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
+__device__ __forceinline__ void sil_packet(const GP &P, int base) {
     using V = typename VT<D>::T;
     __shared__ uint32_t wstack[kBlock / 64][kPacketDepth];
     __shared__ unsigned long long wmask[kBlock / 64][kPacketDepth];
     uint32_t *wst = wstack[threadIdx.x >> 6];
     unsigned long long *wmk = wmask[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = base + threadIdx.x;
     const bool live = i < P.n;
     const uint32_t qi = live ? query_index<D>(P, i) : 0u;
     V x;
@@ -785,7 +868,7 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
             }
             if (!ref_is_leaf(cur)) {
                 PairChild c[2];
-                load_pair<D, Q>(P.nodes, cur, c);
+                load_pair<D, Q>((const void *)P.nodes, cur, c);
                 bool any[2], hh[2];
                 unsigned long long hm[2];
                 float key[2];
@@ -795,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                     box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                     bool h = act && dm <= r;
                     if (h && c[j].ref_cone >= 0) {
-                        const RefCone &rc = P.ref_cones[c[j].ref_cone];
+                        const RefCone rc = ldg(P.ref_cones + c[j].ref_cone);
                         h = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dm);
                     }
                     if (h && exact_cull<D>(c[j], xc)) h = false;
@@ -828,12 +911,12 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                 }
             } else {
                 uint32_t first, count;
-                leaf_range(cur, P.big, first, count);
+                leaf_range(cur, (const uint32_t *)P.big, first, count);
                 first = uniform(first);
                 count = uniform(count);
                 for (uint32_t s = first; s < first + count; ++s) {
                     if constexpr (D == 3) {
-                        const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1], s2 = P.prims[3 * s + 2];
+                        const float4 s0 = ldg(P.prims + (3 * s)), s1 = ldg(P.prims + (3 * s + 1)), s2 = ldg(P.prims + (3 * s + 2));
                         const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
                         V3 pt;
                         const float dd = closest_point_segment(x, a, b, pt);
@@ -847,7 +930,7 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
                             }
                         }
                     } else {
-                        const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
+                        const float4 s0 = ldg(P.prims + (2 * s)), s1 = ldg(P.prims + (2 * s + 1));
                         const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
                         const float dd = norm(x - b);
                         if (act && (dd < r || (dd == r && found && (int)s < best_slot)) &&
@@ -868,7 +951,7 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
         }
     }
     if (!live) return;
-    const gq_records &o = P.out;
+    const GRecords &o = P.out;
     if (!found) {
         if (o.valid) o.valid[qi] = 0;
         if (o.p) for (int k = 0; k < D; ++k) o.p[(size_t)D * qi + k] = 0.0f;
@@ -877,7 +960,7 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
         return;
     }
     if (o.valid) o.valid[qi] = 1;
-    if (o.p) gqd::st(o.p + (size_t)D * qi, best_p);
+    if (o.p) st(o.p + (size_t)D * qi, best_p);
     if (o.d) o.d[qi] = best_d;
     if (o.prim_id) o.prim_id[qi] = P.ids[best_slot];
 }
@@ -885,10 +968,10 @@ __global__ void __launch_bounds__(kBlock) k_silhouette(EngineParams P) {
 // Output record of a closest-silhouette query from the winning silhouette slot (-1: none).  The
 // point and distance are recomputed exactly as the leaf test computed them.
 template <int D>
-__device__ __forceinline__ void sil_record(const EngineParams &P, uint32_t qi, const typename VT<D>::T &x,
+__device__ __forceinline__ void sil_record(const GP &P, uint32_t qi, const typename VT<D>::T &x,
                                            int best_slot) {
     using V = typename VT<D>::T;
-    const gq_records &o = P.out;
+    const GRecords &o = P.out;
     if (best_slot < 0) {
         if (o.valid) o.valid[qi] = 0;
         if (o.p) for (int j = 0; j < D; ++j) o.p[(size_t)D * qi + j] = 0.0f;
@@ -900,15 +983,15 @@ __device__ __forceinline__ void sil_record(const EngineParams &P, uint32_t qi, c
     V bp;
     float bd;
     if constexpr (D == 3) {
-        const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1];
+        const float4 s0 = ldg(P.prims + (3 * s)), s1 = ldg(P.prims + (3 * s + 1));
         bd = closest_point_segment(x, v3(s0.x, s0.y, s0.z), v3(s0.w, s1.x, s1.y), bp);
     } else {
-        const float4 s0 = P.prims[2 * s];
+        const float4 s0 = ldg(P.prims + (2 * s));
         bp = v2(s0.z, s0.w);
         bd = norm(x - bp);
     }
     if (o.valid) o.valid[qi] = 1;
-    if (o.p) gqd::st(o.p + (size_t)D * qi, bp);
+    if (o.p) st(o.p + (size_t)D * qi, bp);
     if (o.d) o.d[qi] = bd;
     if (o.prim_id) o.prim_id[qi] = P.ids[s];
 }
@@ -918,10 +1001,10 @@ __device__ __forceinline__ void sil_record(const EngineParams &P, uint32_t qi, c
 // over the Morton order with a per-lane LDS stack (one entry per binary level at most).  Cheaper
 // than the packet pass when per-query work is small and culling is effective (2D silhouettes).
 template <int D, bool Q>
-__global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
+__device__ __forceinline__ void sil_lane(const GP &P, int base) {
     using V = typename VT<D>::T;
     GQ_LANE_STACK_DECL
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = base + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t qi = query_index<D>(P, i);
     const V x = ldv<D>(P.x + (size_t)D * qi);
@@ -937,7 +1020,7 @@ __global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
     while (go) {
         if (!ref_is_leaf(cur)) {
             PairChild c[2];
-            load_pair<D, Q>(P.nodes, cur, c);
+            load_pair<D, Q>((const void *)P.nodes, cur, c);
             bool h[2];
             float key[2];
 #pragma unroll
@@ -946,7 +1029,7 @@ __global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
                 box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                 bool hj = dm <= r;
                 if (hj && c[j].ref_cone >= 0) {
-                    const RefCone &rc = P.ref_cones[c[j].ref_cone];
+                    const RefCone rc = ldg(P.ref_cones + c[j].ref_cone);
                     float dr;
                     hj = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dr);
                 }
@@ -966,10 +1049,10 @@ __global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
             }
         } else {
             uint32_t first, count;
-            leaf_range(cur, P.big, first, count);
+            leaf_range(cur, (const uint32_t *)P.big, first, count);
             for (uint32_t s = first; s < first + count; ++s) {
                 if constexpr (D == 3) {
-                    const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1], s2 = P.prims[3 * s + 2];
+                    const float4 s0 = ldg(P.prims + (3 * s)), s1 = ldg(P.prims + (3 * s + 1)), s2 = ldg(P.prims + (3 * s + 2));
                     const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
                     V3 pt;
                     const float dd = closest_point_segment(x, a, b, pt);
@@ -980,7 +1063,7 @@ __global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
                         }
                     }
                 } else {
-                    const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
+                    const float4 s0 = ldg(P.prims + (2 * s)), s1 = ldg(P.prims + (2 * s + 1));
                     const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
                     const float dd = norm(x - b);
                     if ((dd < r || (dd == r && best_slot >= 0 && (int)s < best_slot)) &&
@@ -1003,19 +1086,10 @@ __global__ void __launch_bounds__(kRayBlock) k_silhouette_lane(EngineParams P) {
 constexpr int kFrontSil = 3072;
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) {
+__device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, uint2 *front) {
     using V = typename VT<D>::T;
-    __shared__ uint2 front_all[kWaveBlock / 64][kFrontSil];
-    uint2 *front = front_all[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
-    const uint32_t total = uniform(P.defer_count[0]);
-    for (;;) {
-        // All lanes take part in the fetch (lane 0 adds 1, the others 0) and later all lanes store
-        // the same record: no lane-dependent branch inside this loop, so the compiler cannot split
-        // the wavefront across iterations.
-        const uint32_t k = uniform(atomicAdd(P.defer_count + 1, lane == 0 ? 1u : 0u));
-        if (k >= total) break;
-        const uint4 item = P.deferred[k];
+    {
         const uint32_t qi = uniform(item.x);
         float r = uniformf(__uint_as_float(item.y));
         int best_slot = (int)uniform(item.z);
@@ -1041,14 +1115,14 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
             if (lane < take && __uint_as_float(e.y) <= r) {
                 if (!ref_is_leaf(e.x)) {
                     PairChild c[2];
-                    load_pair<D, Q>(P.nodes, e.x, c);
+                    load_pair<D, Q>((const void *)P.nodes, e.x, c);
 #pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         float dm, dx;
                         box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                         bool h = dm <= r;
                         if (h && c[j].ref_cone >= 0) {
-                            const RefCone &rc = P.ref_cones[c[j].ref_cone];
+                            const RefCone rc = ldg(P.ref_cones + c[j].ref_cone);
                             h = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dm);
                         }
                         if (h && exact_cull<D>(c[j], xc)) h = false;
@@ -1058,10 +1132,10 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
                     }
                 } else {
                     uint32_t first, count;
-                    leaf_range(e.x, P.big, first, count);
+                    leaf_range(e.x, (const uint32_t *)P.big, first, count);
                     for (uint32_t s = first; s < first + count; ++s) {
                         if constexpr (D == 3) {
-                            const float4 s0 = P.prims[3 * s], s1 = P.prims[3 * s + 1], s2 = P.prims[3 * s + 2];
+                            const float4 s0 = ldg(P.prims + (3 * s)), s1 = ldg(P.prims + (3 * s + 1)), s2 = ldg(P.prims + (3 * s + 2));
                             const V3 a = v3(s0.x, s0.y, s0.z), b = v3(s0.w, s1.x, s1.y);
                             V3 pt;
                             const float dd = closest_point_segment(x, a, b, pt);
@@ -1073,7 +1147,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
                                 }
                             }
                         } else {
-                            const float4 s0 = P.prims[2 * s], s1 = P.prims[2 * s + 1];
+                            const float4 s0 = ldg(P.prims + (2 * s)), s1 = ldg(P.prims + (2 * s + 1));
                             const V2 a = v2(s0.x, s0.y), b = v2(s0.z, s0.w), cc = v2(s1.x, s1.y);
                             const float dd = norm(x - b);
                             if ((dd < r || (dd == r && best_slot >= 0 && (int)s < best_slot)) &&
@@ -1115,7 +1189,12 @@ __global__ void __launch_bounds__(kWaveBlock) k_silhouette_wave(EngineParams P) 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Morton ordering of the queries
+// Query ordering: a counting sort of each job's queries into Morton-ordered cells of the job's
+// query bounding box (about 8 queries per cell), then an insertion sort of every cell by the full
+// 30-bit (3D) / 32-bit (2D) Morton code, which gives the complete Morton order.  Cells larger than
+// kRefineMax (clustered queries) keep the scatter order, which changes no result: every kernel
+// breaks distance ties to the lowest slot.  Every pass is a plain kernel over a job table: no
+// library call, no memset, no host synchronisation, so a whole batch can be recorded in a graph.
 __device__ __forceinline__ uint32_t f2ord(float f) {
     const uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -1128,56 +1207,14 @@ __global__ void k_zero_u32(uint32_t *b, int n) {
     for (int t = threadIdx.x; t < n; t += blockDim.x) b[t] = 0u;
 }
 
-constexpr int kBoundsBlocks = 256;
-
-// Per-block partial bounds (order-preserving uint encoding) into part[block][6]; no atomics, so
-// the result does not depend on scheduling and the pass can be recorded in a graph.
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_bounds(const float *x, int n, uint32_t *part) {
-    uint32_t mn[D], mx[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) { mn[k] = 0xFFFFFFFFu; mx[k] = 0u; }
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const float f = x[(size_t)D * i + k];
-            if (isfinite(f)) {
-                const uint32_t u = f2ord(f);
-                mn[k] = min(mn[k], u);
-                mx[k] = max(mx[k], u);
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        for (int off = 32; off > 0; off >>= 1) {
-            mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off));
-            mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
-        }
-    }
-    __shared__ uint32_t red[kBlock / 64][6];
-    const int w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        red[w][k] = k < D ? mn[k] : 0xFFFFFFFFu;   // every lane stores the same wave value
-        red[w][3 + k] = k < D ? mx[k] : 0u;
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        const int k = threadIdx.x;
-        uint32_t v = red[0][k];
-        for (int j = 1; j < kBlock / 64; ++j) v = k < 3 ? min(v, red[j][k]) : max(v, red[j][k]);
-        part[blockIdx.x * 6 + k] = v;
-    }
+// Copies one job's parameters into device memory (the single-call path; batches upload a table).
+__global__ void k_store_params(EngineParams P, EngineParams *dst) {
+    if (threadIdx.x == 0) *dst = P;
 }
 
-// Final bounds from the per-block partials: b[0..2] = min, b[3..5] = max.
-__global__ void k_bounds_final(const uint32_t *part, int nblocks, uint32_t *b) {
-    const int k = threadIdx.x;
-    if (k >= 6) return;
-    uint32_t v = k < 3 ? 0xFFFFFFFFu : 0u;
-    for (int j = 0; j < nblocks; ++j) v = k < 3 ? min(v, part[j * 6 + k]) : max(v, part[j * 6 + k]);
-    b[k] = v;
+// Tile t of a launch: (job, first query).  A null table means job 0, tiles of `ts` queries.
+__device__ __forceinline__ uint2 tile_at(const uint2 *tiles, int t, int ts) {
+    return tiles ? tiles[t] : make_uint2(0u, (uint32_t)t * (uint32_t)ts);
 }
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
@@ -1197,173 +1234,494 @@ __device__ __forceinline__ uint32_t spread2(uint32_t v) {  // 16 bits -> every s
     return v;
 }
 
+// (1) partial bounds of each 256-query tile, order-preserving uint encoding, into part[tile][6]
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_morton(const float *x, int n, const uint32_t *b, uint32_t *keys,
-                                                    uint32_t *vals) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    constexpr float kCells = D == 3 ? 1023.0f : 65535.0f;
+__global__ void __launch_bounds__(kBlock) k_sort_bounds(const EngineParams *J, const uint2 *tiles) {
+    const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
+    const EngineParams &P = J[tl.x];
+    const int i = (int)tl.y + threadIdx.x;
+    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    if (i < P.n) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float f = P.x[(size_t)D * i + k];
+            if (isfinite(f)) {
+                mn[k] = f2ord(f);
+                mx[k] = mn[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off));
+            mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
+        }
+    }
+    __shared__ uint32_t red[kBlock / 64][6];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        red[w][k] = mn[k];   // every lane stores the same wave value
+        red[w][3 + k] = mx[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t v = red[0][k];
+        for (int j = 1; j < kBlock / 64; ++j) v = k < 3 ? min(v, red[j][k]) : max(v, red[j][k]);
+        P.part[(tl.y / kBlock) * 6 + k] = v;
+    }
+}
+
+constexpr int kScanBlock = 1024;
+
+// (2) one block per job: final bounds from the tile partials, and zeroed cell counters
+__global__ void __launch_bounds__(kScanBlock) k_sort_final(const EngineParams *J) {
+    const EngineParams &P = J[blockIdx.x];
+    if (!P.perm) return;
+    const int nt = (P.n + kBlock - 1) / kBlock;
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (int t = threadIdx.x; t < nt; t += kScanBlock)
+        for (int k = 0; k < 6; ++k) v[k] = k < 3 ? min(v[k], P.part[t * 6 + k]) : max(v[k], P.part[t * 6 + k]);
+    for (int k = 0; k < 6; ++k)
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)v[k], off);
+            v[k] = k < 3 ? min(v[k], o) : max(v[k], o);
+        }
+    __shared__ uint32_t red[kScanBlock / 64][6];
+    const int w = threadIdx.x >> 6;
+    for (int k = 0; k < 6; ++k) red[w][k] = v[k];
+    for (int c = threadIdx.x; c < P.n_cells; c += kScanBlock) P.counts[c] = 0u;
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t r = red[0][k];
+        for (int j = 1; j < kScanBlock / 64; ++j) r = k < 3 ? min(r, red[j][k]) : max(r, red[j][k]);
+        P.bounds[k] = r;
+    }
+}
+
+// (3) fine Morton code of every query (10 bits per axis in 3D, 16 in 2D) and the histogram of
+// its cell (the top cell_bits per axis)
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_sort_count(const EngineParams *J, const uint2 *tiles) {
+    const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
+    const EngineParams &P = J[tl.x];
+    const int i = (int)tl.y + threadIdx.x;
+    if (i >= P.n) return;
+    constexpr float kLevels = D == 3 ? 1023.0f : 65535.0f;
     uint32_t code = 0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const float lo = ord2f(b[k]), hi = ord2f(b[3 + k]);
+        const float lo = ord2f(P.bounds[k]), hi = ord2f(P.bounds[3 + k]);
         const float ext = hi - lo;
-        float t = ext > 0.0f ? (x[(size_t)D * i + k] - lo) / ext : 0.0f;
+        float t = ext > 0.0f ? (P.x[(size_t)D * i + k] - lo) / ext : 0.0f;
         t = fminf(fmaxf(t, 0.0f), 1.0f);  // NaN -> 0
-        const uint32_t c = (uint32_t)(t * kCells);
+        const uint32_t c = (uint32_t)(t * kLevels);
         code |= (D == 3 ? spread3(c) : spread2(c)) << k;
     }
-    keys[i] = code;
+    P.cell[i] = code;
+    atomicAdd(&P.counts[code >> P.cell_shift], 1u);
+}
+
+// (4) one block per job: exclusive prefix sum of the cell counters, in place
+__global__ void __launch_bounds__(kScanBlock) k_sort_scan(const EngineParams *J) {
+    const EngineParams &P = J[blockIdx.x];
+    if (!P.perm) return;
+    const int ncell = P.n_cells;
+    const int per = (ncell + kScanBlock - 1) / kScanBlock;
+    const int c0 = threadIdx.x * per, c1 = min(c0 + per, ncell);
+    uint32_t sum = 0;
+    for (int c = c0; c < c1; ++c) sum += P.counts[c];
+    // block-wide exclusive scan of the per-thread sums
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+        if (lane >= o) x += y;
+    }
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int j = 0; j < w; ++j) before += wsum[j];
+    uint32_t run = before + x - sum;
+    for (int c = c0; c < c1; ++c) {
+        const uint32_t n = P.counts[c];
+        P.counts[c] = run;
+        run += n;
+    }
+}
+
+// (5) scatter: each query takes the next slot of its cell
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(const EngineParams *J, const uint2 *tiles) {
+    const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
+    const EngineParams &P = J[tl.x];
+    const int i = (int)tl.y + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t pos = atomicAdd(&P.counts[P.cell[i] >> P.cell_shift], 1u);
+    P.perm[pos] = (uint32_t)i;
+}
+
+// Single-call path: fine Morton code and identity value of every query, for a radix sort.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_morton_keys(const EngineParams *J, uint32_t *vals) {
+    const EngineParams &P = J[0];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    constexpr float kLevels = D == 3 ? 1023.0f : 65535.0f;
+    uint32_t code = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const float lo = ord2f(P.bounds[k]), hi = ord2f(P.bounds[3 + k]);
+        const float ext = hi - lo;
+        float t = ext > 0.0f ? (P.x[(size_t)D * i + k] - lo) / ext : 0.0f;
+        t = fminf(fmaxf(t, 0.0f), 1.0f);  // NaN -> 0
+        code |= (D == 3 ? spread3((uint32_t)(t * kLevels)) : spread2((uint32_t)(t * kLevels))) << k;
+    }
+    P.cell[i] = code;
     vals[i] = (uint32_t)i;
+}
+
+constexpr int kRefineMax = 64;
+
+// (6) one thread per cell: insertion sort of the cell's queries by their fine Morton code.  After
+// the scatter, counts[c] is the end of cell c and counts[c - 1] its start.
+__global__ void __launch_bounds__(kBlock) k_sort_refine(const EngineParams *J) {
+    const EngineParams &P = J[blockIdx.y];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (!P.perm || c >= P.n_cells) return;
+    const uint32_t start = c ? P.counts[c - 1] : 0u, end = P.counts[c];
+    if (end - start < 2u || end - start > (uint32_t)kRefineMax) return;
+    for (uint32_t a = start + 1; a < end; ++a) {
+        const uint32_t v = P.perm[a], kv = P.cell[v];
+        uint32_t b = a;
+        while (b > start) {
+            const uint32_t u = P.perm[b - 1];
+            const uint32_t ku = P.cell[u];
+            if (ku < kv || (ku == kv && u < v)) break;
+            P.perm[b] = u;
+            --b;
+        }
+        P.perm[b] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Traversal launches over a job table.  Packet kernels take 256-query tiles, lane kernels 64-query
+// tiles; the wave-per-query passes drain the deferred list of their launch group, whose entries
+// name their job.
+template <int D, bool Q>
+__global__ void __launch_bounds__(kBlock) k_cp_packet(const EngineParams *J, const uint2 *tiles) {
+    const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
+    const GP P = gview(J[tl.x]);
+    cp_packet<D, Q>(P, (int)tl.y);
+}
+
+template <int D, bool Q>
+__global__ void __launch_bounds__(kBlock) k_sil_packet(const EngineParams *J, const uint2 *tiles) {
+    const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
+    const GP P = gview(J[tl.x]);
+    sil_packet<D, Q>(P, (int)tl.y);
+}
+
+template <int D, bool Q>
+__global__ void __launch_bounds__(kRayBlock) k_ray_lane(const EngineParams *J, const uint2 *tiles) {
+    const uint2 tl = tile_at(tiles, blockIdx.x, kRayBlock);
+    const GP P = gview(J[tl.x]);
+    ray_lane<D, Q>(P, (int)tl.y);
+}
+
+template <int D, bool Q>
+__global__ void __launch_bounds__(kRayBlock) k_sil_lane(const EngineParams *J, const uint2 *tiles) {
+    const uint2 tl = tile_at(tiles, blockIdx.x, kRayBlock);
+    const GP P = gview(J[tl.x]);
+    sil_lane<D, Q>(P, (int)tl.y);
+}
+
+// Persistent wave-per-query passes.  All lanes take part in the fetch (lane 0 adds 1, the others
+// 0) and all lanes store the same record: no lane-dependent branch inside the loop, so the
+// compiler cannot split the wavefront across iterations.
+template <int D, bool Q>
+__global__ void __launch_bounds__(kWaveBlock) k_cp_wave(const EngineParams *J, uint32_t *dc, const uint4 *deferred) {
+    __shared__ uint2 front_all[kWaveBlock / 64][kFront];
+    uint2 *front = front_all[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const uint32_t total = uniform(dc[0]);
+    for (;;) {
+        const uint32_t k = uniform(atomicAdd(dc + 1, lane == 0 ? 1u : 0u));
+        if (k >= total) break;
+        const uint4 item = deferred[k];
+        const GP P = gview(J[uniform(item.w)]);
+        cp_wave_item<D, Q>(P, item, front);
+    }
+}
+
+template <int D, bool Q>
+__global__ void __launch_bounds__(kWaveBlock) k_sil_wave(const EngineParams *J, uint32_t *dc, const uint4 *deferred) {
+    __shared__ uint2 front_all[kWaveBlock / 64][kFrontSil];
+    uint2 *front = front_all[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const uint32_t total = uniform(dc[0]);
+    for (;;) {
+        const uint32_t k = uniform(atomicAdd(dc + 1, lane == 0 ? 1u : 0u));
+        if (k >= total) break;
+        const uint4 item = deferred[k];
+        const GP P = gview(J[uniform(item.w)]);
+        sil_wave_item<D, Q>(P, item, front);
+    }
 }
 
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-template <int D, bool Q>
-hipError_t launch_traversal(QueryKind kind, const EngineParams &P, hipStream_t st) {
-    static const char *names[2][3] = {
-        {"engine_closest_point_2d", "engine_intersect_2d", "engine_silhouette_2d"},
-        {"engine_closest_point_3d", "engine_intersect_3d", "engine_silhouette_3d"}};
-    prof_begin(st);
-    if (kind == kIntersect) {
-        const unsigned grid = (unsigned)((P.n + kRayBlock - 1) / kRayBlock);
-        const size_t lds = (size_t)P.stack_depth * kRayBlock * sizeof(uint2);
-        hipLaunchKernelGGL((k_intersect<D, Q>), dim3(grid), dim3(kRayBlock), lds, st, P);
-    } else {
-        const unsigned grid = (unsigned)((P.n + kBlock - 1) / kBlock);
-        if (kind == kClosestPoint) {
-            if (P.budget > 0) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, P.defer_count, 2);
-            hipLaunchKernelGGL((k_closest_point<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
-            if (P.budget > 0)
-                hipLaunchKernelGGL((k_closest_point_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, P);
-        } else if (P.sil_lane) {
-            const unsigned lgrid = (unsigned)((P.n + kRayBlock - 1) / kRayBlock);
-            const size_t lds = (size_t)P.stack_depth * kRayBlock * sizeof(uint2);
-            hipLaunchKernelGGL((k_silhouette_lane<D, Q>), dim3(lgrid), dim3(kRayBlock), lds, st, P);
-        } else {
-            if (P.budget > 0) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, P.defer_count, 2);
-            hipLaunchKernelGGL((k_silhouette<D, Q>), dim3(grid), dim3(kBlock), 0, st, P);
-            if (P.budget > 0)
-                hipLaunchKernelGGL((k_silhouette_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, P);
-        }
-    }
-    prof_end(st, names[D - 2][kind]);
-    return hipGetLastError();
+// ---- host side: plan of a set of jobs -------------------------------------------------------
+// A launch group is one (query kind, dimension, node format, traversal) combination: its jobs
+// share the traversal kernels and one deferred list.
+struct Group {
+    QueryKind kind;
+    int dim;
+    bool quantized;
+    bool lane;                        // lane-per-query kernel (rays, lane silhouettes)
+    std::vector<int> jobs;
+    size_t tiles_off = 0, n_tiles = 0;   // tiles of this group's traversal kernel
+    size_t dc_off = 0, def_off = 0;
+    bool defer = false;
+    int max_stack = 0;
+};
+
+struct Plan {
+    std::vector<EngineParams> P;
+    std::vector<uint2> tiles;         // all tile tables, back to back
+    size_t sort_tiles_off[2] = {0, 0}, n_sort_tiles[2] = {0, 0};   // by dimension (2D, 3D)
+    std::vector<Group> groups;
+    size_t table_bytes = 0;           // job table + tiles (the uploaded head of the workspace)
+    size_t ws_bytes = 0;              // whole device workspace
+};
+
+int cell_bits_for(int64_t n, int dim) {
+    // 4 to 32 queries per cell on average (uniform queries): small enough for the per-cell sort
+    int b = 0;
+    while (b < (dim == 3 ? 7 : 10) && std::pow(2.0, (double)dim * (b + 1)) <= (double)n / 4.0) ++b;
+    return std::max(b, 2);
 }
 
-// Sorts the batch by the Morton code of the query points.  The buffers come from the
-// stream-ordered allocator and are released on the same stream after the traversal, so calls on
-// different streams never share scratch memory.
-template <int D>
-hipError_t sort_queries(const float *x, int n, hipStream_t st, const uint32_t **perm, void **buf) {
-    *perm = nullptr;
-    *buf = nullptr;
-    if (n < kSortMin) return hipSuccess;
-    size_t temp = 0;
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, temp, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                             (uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n, 0,
-                                             D == 3 ? 30 : 32, st);
-    if (e != hipSuccess) return e;
-    const size_t nb = align256((size_t)n * 4);
-    const size_t need = 256 + align256(kBoundsBlocks * 6 * 4) + 4 * nb + align256(temp);
-    if ((e = hipMallocAsync(buf, need, st)) != hipSuccess) return e;
-    uint8_t *base = (uint8_t *)*buf;
-    uint32_t *bounds = (uint32_t *)base;
-    uint32_t *part = (uint32_t *)(base + 256);
-    base += 256 + align256(kBoundsBlocks * 6 * 4);
-    uint32_t *k_in = (uint32_t *)base, *k_out = (uint32_t *)(base + nb);
-    uint32_t *v_in = (uint32_t *)(base + 2 * nb), *v_out = (uint32_t *)(base + 3 * nb);
-    void *tmp = base + 4 * nb;
-    const int rgrid = (int)std::min<int64_t>(kBoundsBlocks, (n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_bounds<D>), dim3(rgrid), dim3(kBlock), 0, st, x, n, part);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, part, rgrid, bounds);
-    hipLaunchKernelGGL((k_morton<D>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, x, n, bounds, k_in, v_in);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    e = rocprim::radix_sort_pairs(tmp, temp, k_in, k_out, v_in, v_out, (size_t)n, 0, D == 3 ? 30 : 32, st);
-    if (e != hipSuccess) return e;
-    *perm = v_out;
+// Fills the plan for `jobs` with every device pointer relative to `base` (null when only the
+// sizes are wanted).
+hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &plan) {
+    plan = Plan{};
+    const size_t nj = jobs.size();
+    plan.P.resize(nj);
+    // groups
+    for (size_t j = 0; j < nj; ++j) {
+        const EngineJob &jb = jobs[j];
+        const WideView &w = *jb.w;
+        const bool sil_lane = jb.kind == kSilhouette &&
+                              (jb.traversal == 2 || (jb.traversal == 0 && w.dim == 2)) && w.bin_depth + 2 <= kRayStackMax;
+        const bool lane = jb.kind == kIntersect || sil_lane;
+        int g = -1;
+        for (size_t k = 0; k < plan.groups.size(); ++k) {
+            const Group &G = plan.groups[k];
+            if (G.kind == jb.kind && G.dim == w.dim && G.quantized == jb.quantized && G.lane == lane) g = (int)k;
+        }
+        if (g < 0) {
+            Group G;
+            G.kind = jb.kind;
+            G.dim = w.dim;
+            G.quantized = jb.quantized;
+            G.lane = lane;
+            plan.groups.push_back(G);
+            g = (int)plan.groups.size() - 1;
+        }
+        plan.groups[g].jobs.push_back((int)j);
+    }
+    // tile tables: sort tiles per dimension, then traversal tiles per group
+    for (int di = 0; di < 2; ++di) {
+        plan.sort_tiles_off[di] = plan.tiles.size();
+        for (size_t j = 0; j < nj; ++j) {
+            if (jobs[j].w->dim != di + 2 || jobs[j].q.n < kSortMin) continue;
+            for (int64_t t = 0; t < jobs[j].q.n; t += kBlock) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
+        }
+        plan.n_sort_tiles[di] = plan.tiles.size() - plan.sort_tiles_off[di];
+    }
+    for (Group &G : plan.groups) {
+        const int ts = G.lane ? kRayBlock : kBlock;
+        G.tiles_off = plan.tiles.size();
+        for (int j : G.jobs)
+            for (int64_t t = 0; t < jobs[j].q.n; t += ts) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
+        G.n_tiles = plan.tiles.size() - G.tiles_off;
+    }
+    // workspace layout: [job table][tiles][per job scratch][per group deferred lists]
+    size_t off = align256(nj * sizeof(EngineParams));
+    const size_t tiles_at = off;
+    off = align256(off + plan.tiles.size() * sizeof(uint2));
+    plan.table_bytes = off;
+    auto at = [&](size_t bytes) {
+        const size_t o = off;
+        off = align256(off + bytes);
+        return base ? (void *)(base + o) : nullptr;
+    };
+    for (size_t j = 0; j < nj; ++j) {
+        const EngineJob &jb = jobs[j];
+        const WideView &w = *jb.w;
+        EngineParams &P = plan.P[j];
+        P = EngineParams{};
+        P.x = jb.q.x;
+        P.v = jb.q.v;
+        P.r_max = jb.q.r_max;
+        P.r_max_scalar = jb.q.r_max_scalar;
+        P.n = (int)jb.q.n;
+        P.out = jb.q.out;
+        P.big = w.big_leaves;
+        P.job = (uint32_t)j;
+        P.stack_depth = 3 * w.wide_depth + 4;
+        if (jb.kind == kSilhouette) {
+            P.nodes = jb.quantized ? w.qsnodes : w.snodes;
+            P.prims = w.sils;
+            P.ids = w.sil_ids;
+            P.ref_cones = w.ref_cones;
+            P.root = w.sroot_ref;
+            for (int k = 0; k < 3; ++k) { P.root_lo[k] = w.sroot_lo[k]; P.root_hi[k] = w.sroot_hi[k]; }
+        } else {
+            P.nodes = jb.quantized ? w.qnodes : w.nodes;
+            P.prims = w.prims;
+            P.ids = w.prim_ids;
+            P.root = w.root_ref;
+        }
+        if (jb.q.n >= kSortMin) {
+            P.cell_bits = cell_bits_for(jb.q.n, w.dim);
+            P.n_cells = 1 << (P.cell_bits * w.dim);
+            P.cell_shift = w.dim * ((w.dim == 3 ? 10 : 16) - P.cell_bits);
+            const size_t ncell = (size_t)P.n_cells;
+            P.perm = (uint32_t *)at((size_t)jb.q.n * 4);
+            P.cell = (uint32_t *)at((size_t)jb.q.n * 4);
+            P.counts = (uint32_t *)at(ncell * 4);
+            P.part = (uint32_t *)at((size_t)((jb.q.n + kBlock - 1) / kBlock) * 24);
+            P.bounds = (uint32_t *)at(32);
+        }
+    }
+    for (Group &G : plan.groups) {
+        size_t nq = 0;
+        for (int j : G.jobs) {
+            EngineParams &P = plan.P[j];
+            const WideView &w = *jobs[j].w;
+            if (G.lane) {
+                if (G.kind == kSilhouette) P.stack_depth = w.bin_depth + 2;
+                P.sil_lane = G.kind == kSilhouette;
+                G.max_stack = std::max(G.max_stack, P.stack_depth);
+                continue;
+            }
+            const bool fits = G.kind == kClosestPoint ? 256 * (w.wide_depth + 1) <= kFront
+                                                      : 128 * (w.bin_depth + 2) <= kFrontSil;
+            const int budget = jobs[j].budget != 0 ? jobs[j].budget : w.packet_budget;
+            if (fits && budget >= 0) {
+                P.budget = budget > 0 ? budget
+                         : G.kind == kSilhouette ? kPacketBudgetSil
+                         : G.dim == 3 ? kPacketBudgetCp3 : kPacketBudgetCp2;
+                G.defer = true;
+                nq += (size_t)jobs[j].q.n;
+            }
+        }
+        if (G.defer) {
+            uint32_t *dc = (uint32_t *)at(256);
+            uint4 *def = (uint4 *)at(nq * sizeof(uint4));
+            G.dc_off = base ? (size_t)((uint8_t *)dc - base) : 0;
+            G.def_off = base ? (size_t)((uint8_t *)def - base) : 0;
+            for (int j : G.jobs) {
+                if (plan.P[j].budget > 0) {
+                    plan.P[j].defer_count = dc;
+                    plan.P[j].deferred = def;
+                }
+            }
+        }
+    }
+    plan.ws_bytes = off;
+    (void)tiles_at;
     return hipSuccess;
 }
 
-template <int D>
-hipError_t launch_dim(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q,
-                      EngineWorkspace &ws, hipStream_t st) {
-    EngineParams P{};
-    P.x = q.x;
-    P.v = q.v;
-    P.r_max = q.r_max;
-    P.r_max_scalar = q.r_max_scalar;
-    P.n = (int)q.n;
-    P.out = q.out;
-    P.big = w.big_leaves;
-    P.stack_depth = 3 * w.wide_depth + 4;
-    if (kind == kSilhouette) {
-        // lane per query wins where culling leaves little work per query (2D); 3D interior queries
-        // need the packet pass and its wave-per-query tail
-        P.sil_lane = (w.traversal == 2 || (w.traversal == 0 && D == 2)) && w.bin_depth + 2 <= kRayStackMax;
-        if (P.sil_lane) P.stack_depth = w.bin_depth + 2;
-        P.nodes = quantized ? w.qsnodes : w.snodes;
-        P.prims = w.sils;
-        P.ids = w.sil_ids;
-        P.ref_cones = w.ref_cones;
-        P.root = w.sroot_ref;
-        for (int k = 0; k < 3; ++k) { P.root_lo[k] = w.sroot_lo[k]; P.root_hi[k] = w.sroot_hi[k]; }
+template <int D, bool Q>
+void launch_group_t(const Group &G, const EngineParams *J, const uint2 *tiles, uint8_t *base, hipStream_t st) {
+    const uint2 *gt = tiles ? tiles + G.tiles_off : nullptr;   // null: implicit tiles of job 0
+    const unsigned nt = (unsigned)G.n_tiles;
+    if (nt == 0) return;
+    uint32_t *dc = G.defer ? (uint32_t *)(base + G.dc_off) : nullptr;
+    const uint4 *def = G.defer ? (const uint4 *)(base + G.def_off) : nullptr;
+    if (G.kind == kIntersect) {
+        hipLaunchKernelGGL((k_ray_lane<D, Q>), dim3(nt), dim3(kRayBlock), (size_t)G.max_stack * kRayBlock * sizeof(uint2), st, J, gt);
+    } else if (G.kind == kSilhouette && G.lane) {
+        hipLaunchKernelGGL((k_sil_lane<D, Q>), dim3(nt), dim3(kRayBlock), (size_t)G.max_stack * kRayBlock * sizeof(uint2), st, J, gt);
+    } else if (G.kind == kSilhouette) {
+        if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 2);
+        hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt);
+        if (dc) hipLaunchKernelGGL((k_sil_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, J, dc, def);
     } else {
-        P.nodes = quantized ? w.qnodes : w.nodes;
-        P.prims = w.prims;
-        P.ids = w.prim_ids;
-        P.root = w.root_ref;
+        if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 2);
+        hipLaunchKernelGGL((k_cp_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt);
+        if (dc) hipLaunchKernelGGL((k_cp_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, J, dc, def);
     }
-    (void)ws;
-    void *buf = nullptr, *dbuf = nullptr;
-    P.budget = 0;
-    hipError_t e = sort_queries<D>(q.x, P.n, st, &P.perm, &buf);
-    const bool defer = (kind == kClosestPoint && 256 * (w.wide_depth + 1) <= kFront) ||
-                       (kind == kSilhouette && !P.sil_lane && 128 * (w.bin_depth + 2) <= kFrontSil);
-    if (e == hipSuccess && defer && w.packet_budget >= 0) {
-        e = hipMallocAsync(&dbuf, 256 + (size_t)P.n * sizeof(uint4), st);
-        if (e == hipSuccess) {
-            P.budget = w.packet_budget > 0 ? w.packet_budget
-                       : kind == kSilhouette  ? kPacketBudgetSil
-                       : D == 3               ? kPacketBudgetCp3
-                                              : kPacketBudgetCp2;
-            P.defer_count = (uint32_t *)dbuf;
-            P.deferred = (uint4 *)((uint8_t *)dbuf + 256);
+}
+
+void launch_group(const Group &G, const EngineParams *J, const uint2 *tiles, uint8_t *base, hipStream_t st) {
+    if (G.dim == 3) {
+        if (G.quantized) launch_group_t<3, true>(G, J, tiles, base, st);
+        else launch_group_t<3, false>(G, J, tiles, base, st);
+    } else {
+        if (G.quantized) launch_group_t<2, true>(G, J, tiles, base, st);
+        else launch_group_t<2, false>(G, J, tiles, base, st);
+    }
+}
+
+// Enqueues the sort passes and every group's traversal.  `tiles_null` lets a single job use the
+// implicit tile table (tile_at with a null table) so the single-call path uploads nothing.
+void launch_groups(const Plan &plan, uint8_t *base, const uint2 *tiles, hipStream_t st, const char *prof_name) {
+    const EngineParams *J = (const EngineParams *)base;
+    if (prof_name) prof_begin(st);
+    for (const Group &G : plan.groups) launch_group(G, J, tiles, base, st);
+    if (prof_name) prof_end(st, prof_name);
+}
+
+// Counting sort of every job with at least kSortMin queries, then the traversal of every group.
+hipError_t launch_plan(const Plan &plan, uint8_t *base, bool implicit_tiles, hipStream_t st, const char *prof_name) {
+    if (implicit_tiles && (plan.P.size() != 1 || plan.groups.size() != 1)) return hipErrorInvalidValue;
+    const EngineParams *J = (const EngineParams *)base;
+    const uint2 *tiles = implicit_tiles ? nullptr : (const uint2 *)(base + align256(plan.P.size() * sizeof(EngineParams)));
+    const int nj = (int)plan.P.size();
+    const bool any_sort = plan.n_sort_tiles[0] + plan.n_sort_tiles[1] > 0;
+    if (any_sort) {
+        for (int di = 0; di < 2; ++di) {
+            const unsigned nt = (unsigned)plan.n_sort_tiles[di];
+            if (!nt) continue;
+            const uint2 *t = tiles ? tiles + plan.sort_tiles_off[di] : nullptr;
+            if (di == 0) hipLaunchKernelGGL(k_sort_bounds<2>, dim3(nt), dim3(kBlock), 0, st, J, t);
+            else hipLaunchKernelGGL(k_sort_bounds<3>, dim3(nt), dim3(kBlock), 0, st, J, t);
         }
+        hipLaunchKernelGGL(k_sort_final, dim3(nj), dim3(kScanBlock), 0, st, J);
+        for (int di = 0; di < 2; ++di) {
+            const unsigned nt = (unsigned)plan.n_sort_tiles[di];
+            if (!nt) continue;
+            const uint2 *t = tiles ? tiles + plan.sort_tiles_off[di] : nullptr;
+            if (di == 0) hipLaunchKernelGGL(k_sort_count<2>, dim3(nt), dim3(kBlock), 0, st, J, t);
+            else hipLaunchKernelGGL(k_sort_count<3>, dim3(nt), dim3(kBlock), 0, st, J, t);
+        }
+        hipLaunchKernelGGL(k_sort_scan, dim3(nj), dim3(kScanBlock), 0, st, J);
+        for (int di = 0; di < 2; ++di) {
+            const unsigned nt = (unsigned)plan.n_sort_tiles[di];
+            if (!nt) continue;
+            const uint2 *t = tiles ? tiles + plan.sort_tiles_off[di] : nullptr;
+            hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(kBlock), 0, st, J, t);
+        }
+        int max_cells = 0;
+        for (const EngineParams &P : plan.P) max_cells = std::max(max_cells, P.n_cells);
+        if (max_cells > 0)
+            hipLaunchKernelGGL(k_sort_refine, dim3((max_cells + kBlock - 1) / kBlock, nj), dim3(kBlock), 0, st, J);
     }
-    if (e == hipSuccess)
-        e = quantized ? launch_traversal<D, true>(kind, P, st) : launch_traversal<D, false>(kind, P, st);
-    if (dbuf) {
-        hipError_t f = hipFreeAsync(dbuf, st);
-        if (e == hipSuccess) e = f;
-    }
-    if (buf) {
-        hipError_t f = hipFreeAsync(buf, st);
-        if (e == hipSuccess) e = f;
-    }
-    return e;
+    launch_groups(plan, base, tiles, st, prof_name);
+    return hipGetLastError();
 }
 
 } // namespace
-
-hipError_t EngineWorkspace::ensure(size_t need) {
-    if (need <= bytes) return hipSuccess;
-    if (buf) {
-        // earlier calls on this scene may still read the old buffer
-        hipError_t e = hipDeviceSynchronize();
-        if (e != hipSuccess) return e;
-    }
-    release();
-    hipError_t e = hipMalloc(&buf, need);
-    if (e == hipSuccess) bytes = need;
-    return e;
-}
-
-void EngineWorkspace::release() {
-    if (buf) hipFree(buf);
-    buf = nullptr;
-    bytes = 0;
-}
 
 bool engine_supported(QueryKind kind, const WideView &w) {
     if (kind == kSilhouette) return w.bin_depth + 2 <= kPacketDepth;
@@ -1371,10 +1729,75 @@ bool engine_supported(QueryKind kind, const WideView &w) {
     return 3 * w.wide_depth + 4 <= kRayStackMax;
 }
 
-hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q,
-                         EngineWorkspace &ws, hipStream_t st) {
+hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q, hipStream_t st) {
+    static const char *names[2][3] = {
+        {"engine_closest_point_2d", "engine_intersect_2d", "engine_silhouette_2d"},
+        {"engine_closest_point_3d", "engine_intersect_3d", "engine_silhouette_3d"}};
     if (q.n <= 0) return hipSuccess;
-    return w.dim == 2 ? launch_dim<2>(kind, w, quantized, q, ws, st) : launch_dim<3>(kind, w, quantized, q, ws, st);
+    const std::vector<EngineJob> jobs{EngineJob{&w, kind, quantized, q, 0, w.traversal}};
+    Plan plan;
+    hipError_t e = make_plan(jobs, nullptr, plan);
+    if (e != hipSuccess) return e;
+    // A single call sorts with a device radix sort of the full Morton codes (fastest for one
+    // large batch); batches use the counting sort, which records into graphs.
+    const int n = (int)q.n;
+    const bool sorted = n >= kSortMin;
+    size_t temp = 0;
+    if (sorted) {
+        e = rocprim::radix_sort_pairs(nullptr, temp, (uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                      (uint32_t *)nullptr, (size_t)n, 0, w.dim == 3 ? 30 : 32, st);
+        if (e != hipSuccess) return e;
+    }
+    const size_t nb = align256((size_t)n * 4);
+    const size_t extra = sorted ? 2 * nb + align256(temp) : 0;
+    // stream-ordered scratch: calls on different streams never share it
+    void *buf = nullptr;
+    if ((e = hipMallocAsync(&buf, plan.ws_bytes + extra, st)) != hipSuccess) return e;
+    uint8_t *base = (uint8_t *)buf;
+    make_plan(jobs, base, plan);
+    const EngineParams &P = plan.P[0];
+    hipLaunchKernelGGL(k_store_params, dim3(1), dim3(64), 0, st, P, (EngineParams *)base);
+    const EngineParams *J = (const EngineParams *)base;
+    if (sorted) {
+        uint32_t *keys_out = (uint32_t *)(base + plan.ws_bytes), *vals_in = (uint32_t *)(base + plan.ws_bytes + nb);
+        void *tmp = base + plan.ws_bytes + 2 * nb;
+        const unsigned nt = (unsigned)((n + kBlock - 1) / kBlock);
+        if (w.dim == 3) hipLaunchKernelGGL(k_sort_bounds<3>, dim3(nt), dim3(kBlock), 0, st, J, nullptr);
+        else hipLaunchKernelGGL(k_sort_bounds<2>, dim3(nt), dim3(kBlock), 0, st, J, nullptr);
+        hipLaunchKernelGGL(k_sort_final, dim3(1), dim3(kScanBlock), 0, st, J);
+        if (w.dim == 3) hipLaunchKernelGGL(k_morton_keys<3>, dim3(nt), dim3(kBlock), 0, st, J, vals_in);
+        else hipLaunchKernelGGL(k_morton_keys<2>, dim3(nt), dim3(kBlock), 0, st, J, vals_in);
+        if ((e = hipGetLastError()) == hipSuccess)
+            e = rocprim::radix_sort_pairs(tmp, temp, P.cell, keys_out, vals_in, P.perm, (size_t)n, 0,
+                                          w.dim == 3 ? 30 : 32, st);
+    }
+    if (e == hipSuccess) {
+        launch_groups(plan, base, nullptr, st, names[w.dim - 2][kind]);
+        e = hipGetLastError();
+    }
+    const hipError_t f = hipFreeAsync(buf, st);
+    return e != hipSuccess ? e : f;
+}
+
+hipError_t engine_batch_sizes(const std::vector<EngineJob> &jobs, size_t &ws_bytes, size_t &table_bytes) {
+    Plan plan;
+    const hipError_t e = make_plan(jobs, nullptr, plan);
+    ws_bytes = plan.ws_bytes;
+    table_bytes = plan.table_bytes;
+    return e;
+}
+
+hipError_t launch_engine_batch(const std::vector<EngineJob> &jobs, void *ws, size_t ws_bytes, void *stage,
+                               size_t stage_bytes, hipStream_t st) {
+    Plan plan;
+    hipError_t e = make_plan(jobs, (uint8_t *)ws, plan);
+    if (e != hipSuccess) return e;
+    if (plan.ws_bytes > ws_bytes || plan.table_bytes > stage_bytes) return hipErrorInvalidValue;
+    uint8_t *h = (uint8_t *)stage;
+    std::memcpy(h, plan.P.data(), plan.P.size() * sizeof(EngineParams));
+    std::memcpy(h + align256(plan.P.size() * sizeof(EngineParams)), plan.tiles.data(), plan.tiles.size() * sizeof(uint2));
+    if ((e = hipMemcpyAsync(ws, stage, plan.table_bytes, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+    return launch_plan(plan, (uint8_t *)ws, false, st, nullptr);
 }
 
 } // namespace gqa

@@ -23,7 +23,6 @@ struct gq_scene {
     std::vector<void *> allocs;
     gqa::PlainView plain{};
     gqa::WideView wide{};
-    gqa::EngineWorkspace ws;
 };
 
 namespace gqa {
@@ -198,7 +197,6 @@ int gq_scene_destroy(gq_scene *scene) {
     hipGetDevice(&dev);
     hipSetDevice(scene->device);
     for (void *p : scene->allocs) hipFree(p);
-    scene->ws.release();
     hipSetDevice(dev);
     delete scene;
     return GQ_OK;
@@ -306,7 +304,7 @@ static int run_query(gq_scene *s, QueryKind kind, int method, const QueryArgs &q
     } else if (method == GQ_METHOD_ENGINE || method == GQ_METHOD_ENGINE_FP32) {
         if (!engine_supported(kind, s->wide))
             return fail(GQ_ERR_UNSUPPORTED, "tree deeper than the engine kernels' traversal stack");
-        e = launch_engine(kind, s->wide, method == GQ_METHOD_ENGINE, q, s->ws, st);
+        e = launch_engine(kind, s->wide, method == GQ_METHOD_ENGINE, q, st);
     } else {
         return fail(GQ_ERR_INVALID, "unknown method");
     }

@@ -161,17 +161,27 @@ struct WideHost {
 
 void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims);
 
-// Device scratch owned by a scene (Morton keys, permutation, radix-sort temp).  Grown on demand
-// with hipMalloc; reused by later calls (calls on one scene are ordered on their streams).
-struct EngineWorkspace {
-    void *buf = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t need);
-    void release();
+// One query batch for the engine: a scene's device layout, the query kind and node format, the
+// caller's arrays, and per-batch overrides (0 = the scene's setting).
+struct EngineJob {
+    const WideView *w;
+    QueryKind kind;
+    bool quantized;
+    QueryArgs q;
+    int budget;
+    int traversal;
 };
 
-hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q,
-                         EngineWorkspace &ws, hipStream_t st);
+// One batch on `st`: scratch comes from the stream-ordered allocator and is released after use.
+hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q, hipStream_t st);
+// Many batches (any scenes, kinds and sizes on one device) in one enqueue: sort passes over all of
+// them, then one traversal launch per (kind, dimension, node format, traversal) group.  The job
+// table and tile lists are staged in `stage` (pinned host memory, `table_bytes` long, left
+// untouched until the stream has passed this call) and copied to the head of `ws`.  Nothing is
+// allocated or cleared with memset, so the enqueue can be captured into a graph.
+hipError_t engine_batch_sizes(const std::vector<EngineJob> &jobs, size_t &ws_bytes, size_t &table_bytes);
+hipError_t launch_engine_batch(const std::vector<EngineJob> &jobs, void *ws, size_t ws_bytes, void *stage,
+                               size_t stage_bytes, hipStream_t st);
 // Whether the engine's fixed-capacity traversal stack covers this scene's tree depth.
 bool engine_supported(QueryKind kind, const WideView &w);
 

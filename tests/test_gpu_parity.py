@@ -210,3 +210,21 @@ def test_silhouette_traversal_modes(gpu, traversal, name, method):
     gap = _gap_from_bruteforce("sil", dim, shape.snch, V, F, x, None, ref)
     res = parity.summarize(f"{name}/sil/{method}/traversal{traversal}", got, ref, gap, vec_fields=("p",))
     assert res["ok"], res
+
+
+@pytest.mark.parametrize("name", ["star4096", "ico4"])
+@pytest.mark.parametrize("kind", ["cp", "ray", "sil"])
+@pytest.mark.parametrize("method", ["engine_fp32", "engine"])
+def test_sorted_batch(gpu, name, kind, method):
+    """Batches of 8,192 queries and more go through the Morton-cell counting sort first."""
+    V, F, shape = scene(name)
+    dim = V.shape[1]
+    n = 20000
+    x = synth.uniform_points(n, dim, seed=51)
+    v = synth.uniform_directions(n, dim, seed=52)
+    ref = _oracle(kind, dim, shape.snch, x, v)
+    got = _np(_query(shape, kind, method, x, v))
+    gap = _gap_from_bruteforce(kind, dim, shape.snch, V, F, x, v, ref)
+    vec = ("p", "n") if kind != "sil" else ("p",)
+    res = parity.summarize(f"{name}/{kind}/{method}/sorted", got, ref, gap, vec_fields=vec)
+    assert res["ok"], res
