@@ -39,6 +39,30 @@ class Records(C.Structure):
     ]
 
 
+QUERY_CLOSEST_POINT = 0
+QUERY_INTERSECT = 1
+QUERY_SILHOUETTE = 2
+QUERIES = {"cp": QUERY_CLOSEST_POINT, "ray": QUERY_INTERSECT, "sil": QUERY_SILHOUETTE}
+
+
+class Job(C.Structure):
+    """gq_job: one query batch of a gq_batch_enqueue call."""
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("kind", C.c_int32),
+        ("method", C.c_int32),
+        ("x", C.c_void_p),
+        ("v", C.c_void_p),
+        ("r_max", C.c_void_p),
+        ("r_max_scalar", C.c_float),
+        ("packet_budget", C.c_int32),
+        ("n", C.c_int64),
+        ("out", Records),
+        ("traversal", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
 _i64p = C.POINTER(C.c_int64)
 _fp = C.POINTER(C.c_float)
 
@@ -64,6 +88,8 @@ SIGNATURES = {
     "gq_intersect_bvh": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_closest_silhouette_snch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_structure_engine_layout": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, _i64p]),
+    "gq_batch_sizes": (C.c_int, [C.c_int, C.POINTER(Job), _i64p, _i64p]),
+    "gq_batch_enqueue": (C.c_int, [C.c_int, C.POINTER(Job), C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]),
     "gq_profile_begin": (C.c_int, []),
     "gq_profile_end": (C.c_int, [C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
 }

@@ -334,6 +334,62 @@ int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, cons
     return run_query(scene, kSilhouette, method, q, stream);
 }
 
+static int batch_jobs(int n_jobs, const gq_job *jobs, std::vector<EngineJob> &out) {
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) return fail(GQ_ERR_INVALID, "bad job list");
+    out.clear();
+    out.reserve((size_t)n_jobs);
+    int device = -1;
+    for (int j = 0; j < n_jobs; ++j) {
+        const gq_job &b = jobs[j];
+        const std::string at = "job " + std::to_string(j) + ": ";
+        if (!b.scene) return fail(GQ_ERR_INVALID, at + "null scene");
+        if (b.kind < GQ_QUERY_CLOSEST_POINT || b.kind > GQ_QUERY_SILHOUETTE) return fail(GQ_ERR_INVALID, at + "unknown query kind");
+        if (b.method != GQ_METHOD_ENGINE && b.method != GQ_METHOD_ENGINE_FP32)
+            return fail(GQ_ERR_UNSUPPORTED, at + "batches run the engine methods only");
+        if (b.n < 0 || b.n > (int64_t)INT32_MAX) return fail(GQ_ERR_INVALID, at + "bad query count");
+        const QueryKind kind = (QueryKind)b.kind;
+        if (b.n > 0 && (!b.x || (kind == kIntersect && !b.v))) return fail(GQ_ERR_INVALID, at + "null query pointer");
+        if (kind == kSilhouette && !b.scene->has_snch) return fail(GQ_ERR_UNSUPPORTED, at + "silhouettes need an SNCH scene");
+        if (!engine_supported(kind, b.scene->wide)) return fail(GQ_ERR_UNSUPPORTED, at + "tree deeper than the engine's stack");
+        if (b.traversal < 0 || b.traversal > 2) return fail(GQ_ERR_INVALID, at + "traversal must be 0, 1 or 2");
+        if (device >= 0 && b.scene->device != device) return fail(GQ_ERR_INVALID, at + "scenes on different devices");
+        device = b.scene->device;
+        if (b.n == 0) continue;
+        QueryArgs q{b.x, b.v, b.r_max, b.r_max_scalar, b.n, b.out};
+        out.push_back(EngineJob{&b.scene->wide, kind, b.method == GQ_METHOD_ENGINE, q, b.packet_budget,
+                                b.traversal != 0 ? b.traversal : b.scene->wide.traversal});
+    }
+    return GQ_OK;
+}
+
+int gq_batch_sizes(int n_jobs, const gq_job *jobs, int64_t *device_bytes, int64_t *host_bytes) {
+    if (!device_bytes || !host_bytes) return fail(GQ_ERR_INVALID, "null argument");
+    std::vector<EngineJob> ej;
+    const int r = batch_jobs(n_jobs, jobs, ej);
+    if (r != GQ_OK) return r;
+    size_t ws = 0, table = 0;
+    if (!ej.empty()) engine_batch_sizes(ej, ws, table);
+    *device_bytes = (int64_t)ws;
+    *host_bytes = (int64_t)table;
+    return GQ_OK;
+}
+
+int gq_batch_enqueue(int n_jobs, const gq_job *jobs, void *device_ws, int64_t device_bytes, void *host_stage,
+                     int64_t host_bytes, void *stream) {
+    std::vector<EngineJob> ej;
+    const int r = batch_jobs(n_jobs, jobs, ej);
+    if (r != GQ_OK) return r;
+    if (ej.empty()) return GQ_OK;
+    size_t ws = 0, table = 0;
+    engine_batch_sizes(ej, ws, table);
+    if (!device_ws || !host_stage || device_bytes < (int64_t)ws || host_bytes < (int64_t)table)
+        return fail(GQ_ERR_INVALID, "workspace or staging buffer smaller than gq_batch_sizes reports");
+    const hipError_t e = launch_engine_batch(ej, device_ws, (size_t)device_bytes, host_stage, (size_t)host_bytes,
+                                             (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "batch launch");
+    return GQ_OK;
+}
+
 int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int64_t *n_bytes) {
     if (!st || !n_bytes) return fail(GQ_ERR_INVALID, "null argument");
     WideHost wh;

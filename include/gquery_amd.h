@@ -124,6 +124,40 @@ int gq_intersect_bvh(gq_scene *scene, int method, const float *x, const float *v
 int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, const float *r_max,
                                float r_max_scalar, int64_t n, const gq_records *out, void *stream);
 
+/* ---- Batches (no counterpart in the reference, which runs one DrJit-traced query per call; this
+ * is the batched path for many scenes and query batches in one enqueue) ---- */
+
+#define GQ_QUERY_CLOSEST_POINT 0
+#define GQ_QUERY_INTERSECT 1
+#define GQ_QUERY_SILHOUETTE 2
+
+typedef struct gq_job {
+    gq_scene *scene;         /* any scene on the stream's device                               */
+    int32_t kind;            /* GQ_QUERY_*                                                      */
+    int32_t method;          /* GQ_METHOD_ENGINE_FP32 or GQ_METHOD_ENGINE                       */
+    const float *x;          /* n x dim query points / ray origins                              */
+    const float *v;          /* rays: n x dim directions; else NULL                             */
+    const float *r_max;      /* rays and silhouettes: n limits, or NULL for r_max_scalar        */
+    float r_max_scalar;
+    int32_t packet_budget;   /* 0 = the scene's setting (gq_scene_options)                      */
+    int64_t n;
+    gq_records out;          /* as for the single-batch calls of the same kind                  */
+    int32_t traversal;       /* 0 = the scene's setting                                         */
+    int32_t reserved[3];
+} gq_job;
+
+/* Bytes of device workspace and of pinned host staging that gq_batch_enqueue needs for `jobs`. */
+int gq_batch_sizes(int n_jobs, const gq_job *jobs, int64_t *device_bytes, int64_t *host_bytes);
+
+/* Enqueues every job on `stream`: the job table is written into `host_stage` (pinned host memory,
+ * left untouched until the stream has passed this call) and copied into `device_ws`, then one set
+ * of sort passes covers all jobs and one traversal launch runs per (kind, dimension, node format,
+ * traversal) group.  Allocates nothing, clears nothing with memset and never synchronises, so the
+ * call can be recorded in a HIP graph and replayed (replays re-read host_stage and the job arrays).
+ * Results equal those of the single-batch calls bit for bit. */
+int gq_batch_enqueue(int n_jobs, const gq_job *jobs, void *device_ws, int64_t device_bytes,
+                     void *host_stage, int64_t host_bytes, void *stream);
+
 /* ---- Engine layout inspection (no counterpart in the reference: exposes the host-built engine
  * arrays so tests can check them on machines without a GPU) ---- */
 

@@ -1,0 +1,79 @@
+"""GPU: batched enqueue (gq_batch_enqueue) against the single-batch calls, and graph replays."""
+import numpy as np
+import pytest
+import torch
+
+from gquery_amd import synth
+from gquery_amd.batch import Batch
+from tests.test_gpu_parity import scene
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("valid", "p", "n", "uv", "d", "prim_id", "on_boundary")
+
+
+def _np(r):
+    torch.cuda.synchronize()
+    return {f: getattr(r, f).cpu().numpy() for f in FIELDS if hasattr(r, f)}
+
+
+def _single(shape, kind, x, v, method):
+    if kind == "cp":
+        return shape.closest_point_bvh(x, method=method)
+    if kind == "ray":
+        return shape.intersect_bvh(x, v, method=method)
+    return shape.closest_silhouette_snch(x, method=method)
+
+
+def _jobs(seed=0):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k, (name, kind, n) in enumerate([("star4096", "cp", 20000), ("ico4", "ray", 3000), ("oico4", "sil", 12000),
+                                          ("ico4", "cp", 9000), ("star4096", "sil", 5000), ("ostar1024", "ray", 16000),
+                                          ("ico3", "sil", 30000), ("star1024", "cp", 100)]):
+        V, F, shape = scene(name)
+        dim = V.shape[1]
+        x = torch.from_numpy(synth.uniform_points(n, dim, seed=100 + k)).cuda()
+        v = torch.from_numpy(synth.uniform_directions(n, dim, seed=200 + k)).cuda() if kind == "ray" else None
+        jobs.append((shape, kind, x, v))
+    return jobs
+
+
+@pytest.mark.parametrize("method", ["engine", "engine_fp32"])
+def test_batch_equals_single_calls(gpu, method):
+    jobs = _jobs()
+    b = Batch(jobs, method=method)
+    got = b.run()
+    for (shape, kind, x, v), r in zip(jobs, got):
+        ref = _np(_single(shape, kind, x, v, method))
+        g = _np(r)
+        for f in ref:
+            assert np.array_equal(g[f].view(np.uint8), ref[f].view(np.uint8)), (kind, f)
+
+
+def test_batch_graph_replays(gpu):
+    jobs = _jobs(1)
+    b = Batch(jobs)
+    b.capture()
+    for it in range(3):
+        # new inputs in place, eager work in between, then a replay must match single calls
+        for k, (shape, kind, x, v) in enumerate(jobs):
+            dim = x.shape[1]
+            x.copy_(torch.from_numpy(synth.uniform_points(x.shape[0], dim, seed=1000 * it + k)))
+        torch.cuda.synchronize()
+        got = b.replay()
+        torch.cuda.synchronize()
+        for (shape, kind, x, v), r in zip(jobs, got):
+            ref = _np(_single(shape, kind, x, v, "engine"))
+            g = _np(r)
+            for f in ref:
+                assert np.array_equal(g[f].view(np.uint8), ref[f].view(np.uint8)), (it, kind, f)
+
+
+def test_batch_rejects_bad_jobs(gpu):
+    V, F, shape = scene("star1024")
+    x = torch.zeros((10, 2), device="cuda:0")
+    with pytest.raises(NotImplementedError):
+        Batch([(shape, "cp", x)], method="plain")
+    with pytest.raises(ValueError):
+        Batch([{"shape": shape, "kind": "cp", "x": x, "traversal": 7}])
