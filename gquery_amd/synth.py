@@ -138,3 +138,40 @@ def uniform_directions(n: int, dim: int, seed: int = DIRECTION_SEED):
     else:
         raise ValueError("dim must be 2 or 3")
     return d.astype(np.float32)
+
+
+WORKLOAD_SEED = 7
+
+
+def batched_workload(n_scenes: int = 64, scale: float = 1.0, seed: int = WORKLOAD_SEED):
+    """Scene and batch specs of the uneven batched workload: half 2D star polylines, half 3D
+    displaced icospheres, primitive counts log-uniform in [1k, 2M] (3D: the subdivision level
+    nearest in log), one batch per scene with a size log-uniform in [4k, 4M] and a query kind
+    uniform over closest point / ray / silhouette.  `scale` multiplies both ranges (for smaller
+    runs).  Scene k uses geometry seed GEOMETRY_SEED + 1000 (k + 1) and query seed
+    QUERY_SEED + 1000 (k + 1)."""
+    rng = _rng(seed)
+    dims = np.array([2] * (n_scenes // 2) + [3] * (n_scenes - n_scenes // 2))
+    rng.shuffle(dims)
+    specs = []
+    for k in range(n_scenes):
+        target = float(np.exp(rng.uniform(np.log(1e3), np.log(2e6)))) * scale
+        nq = max(int(np.exp(rng.uniform(np.log(4e3), np.log(4e6))) * scale), 1)
+        kind = ["cp", "ray", "sil"][int(rng.integers(0, 3))]
+        if dims[k] == 3:
+            levels = np.arange(1, 9)
+            s = int(levels[np.argmin(np.abs(np.log(20.0 * 4.0 ** levels) - np.log(max(target, 20.0))))])
+            spec = dict(dim=3, level=s, prims=20 * 4 ** s)
+        else:
+            spec = dict(dim=2, segments=max(int(round(target)), 3), prims=max(int(round(target)), 3))
+        spec.update(geometry_seed=GEOMETRY_SEED + 1000 * (k + 1), query_seed=QUERY_SEED + 1000 * (k + 1),
+                    queries=nq, kind=kind)
+        specs.append(spec)
+    return specs
+
+
+def scene_geometry(spec):
+    """(vertices, indices) of a workload scene spec."""
+    if spec["dim"] == 3:
+        return displaced_icosphere(spec["level"], seed=spec["geometry_seed"])
+    return star_polyline(spec["segments"], seed=spec["geometry_seed"])
