@@ -257,6 +257,82 @@ def batched(args):
     print(json.dumps(out))
 
 
+def served(args, loads=(0.25, 0.5, 0.9)):
+    """Serving workload: 16 scenes resident, requests (scene, kind 60/25/15 % cp/ray/sil, 256 to
+    262,144 queries) admitted by free device memory and coalesced into batches on two stream
+    slots.  The saturated rate is measured with every request present at once; then Poisson
+    streams at fractions of it give p50 / p99 latency (completion - arrival) and throughput."""
+    import concurrent.futures as cf
+    import numpy as np
+    import torch
+    from gquery_amd import synth
+    from gquery_amd.serve import GpuExecutor, Request, Scheduler, percentile, run_open_loop
+    from gquery_amd.shapes.mesh import Mesh
+    from gquery_amd.shapes.polyline import Polyline
+    torch.cuda.set_device(0)
+    scene_specs, req_specs = synth.served_workload(n_requests=args.requests, scale=args.scale)
+    t0 = time.perf_counter()
+
+    def make(sp):
+        V, F = synth.scene_geometry(sp)
+        torch.cuda.set_device(0)
+        return (Mesh if sp["dim"] == 3 else Polyline)(V, F, device="cuda:0")
+
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        shapes = list(ex.map(make, scene_specs))
+    inputs = []
+    for q in req_specs:
+        dim = scene_specs[q["scene"]]["dim"]
+        x = torch.from_numpy(synth.uniform_points(q["n"], dim, seed=q["query_seed"])).cuda()
+        v = torch.from_numpy(synth.uniform_directions(q["n"], dim, seed=q["query_seed"] + 1)).cuda() if q["kind"] == "ray" else None
+        inputs.append((x, v))
+    torch.cuda.synchronize()
+    n_q = sum(q["n"] for q in req_specs)
+    log(f"served: {len(shapes)} scenes ({sum(sp['prims'] for sp in scene_specs) / 1e6:.2f}M primitives) built in "
+        f"{time.perf_counter() - t0:.0f}s; {len(req_specs)} requests, {n_q / 1e6:.2f}M queries per run")
+    slots = (2, 1)
+    executor = GpuExecutor(shapes, slots=slots, method=args.method if args.method != "plain" else "engine")
+    budget = int(torch.cuda.mem_get_info()[0] * 0.5)
+
+    def run(arrivals):
+        reqs = [Request(rid=i, scene=q["scene"], kind=q["kind"], x=inputs[i][0], v=inputs[i][1])
+                for i, q in enumerate(req_specs)]
+        launches0 = executor.launches
+        sched = Scheduler(executor, memory_budget=budget, slots=slots)
+        run_open_loop(sched, reqs, arrivals)
+        torch.cuda.synchronize()
+        lat = [r.latency * 1e3 for r in reqs]
+        fast = [r.latency * 1e3 for r in reqs if r.kind != "sil"]
+        sil = [r.latency * 1e3 for r in reqs if r.kind == "sil"]
+        span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+        return {"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "mean_ms": float(np.mean(lat)),
+                "cp_ray_p50_ms": percentile(fast, 50), "cp_ray_p99_ms": percentile(fast, 99),
+                "sil_p50_ms": percentile(sil, 50), "sil_p99_ms": percentile(sil, 99),
+                "Mq_per_s": n_q / span / 1e6, "launches": executor.launches - launches0}
+
+    run([0.0] * len(req_specs))                       # warm-up (allocator, code objects)
+    sat = run([0.0] * len(req_specs))
+    sat_qps = sat["Mq_per_s"] * 1e6
+    out = {"saturated": sat}
+    rng = np.random.default_rng(synth.WORKLOAD_SEED + 2)
+    mean_n = n_q / len(req_specs)
+    for load in loads:
+        lam = load * sat_qps / mean_n                   # requests per second
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(req_specs)))
+        out[f"load_{int(load * 100)}"] = run(list(arrivals - arrivals[0]))
+        log(f"load {load:.2f}: {out[f'load_{int(load * 100)}']}")
+    p99 = float(out["load_90"]["p99_ms"]) if "load_90" in out else None
+    for k in out:
+        out[k] = {a: float(b) for a, b in out[k].items()}
+    return {"metric": "served p99 latency (ms) at 90% of saturated load", "value": p99, "unit": "ms",
+            "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic: seeded scenes and Poisson request stream",
+            "config": {"workload": f"served: {len(shapes)} scenes, {len(req_specs)} requests per load, "
+                                   f"{n_q} queries, memory budget {budget} B, stream slots {slots} (cp+ray, sil)",
+                       "scale": args.scale},
+            "served": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,7 +363,14 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="uneven batched workload: 64 scenes, one batch each, one graph replay per step")
     ap.add_argument("--scale", type=float, default=1.0, help="--batched / --served: size multiplier")
+    ap.add_argument("--served", action="store_true",
+                    help="serving loop: 16 resident scenes, Poisson request stream at 25/50/90%% of the "
+                         "saturated rate, p50/p99 latency")
+    ap.add_argument("--requests", type=int, default=400, help="--served: requests per load level")
     args = ap.parse_args()
+    if args.served:
+        print(json.dumps(served(args)))
+        return
     if args.batched:
         batched(args)
         return

@@ -33,7 +33,7 @@ using namespace gqd;
 constexpr int kBlock = 256;     // packet kernels: 4 wavefronts per block
 constexpr int kRayBlock = 64;   // lane-per-query ray kernel: one wavefront per block
 constexpr int kRayStackMax = 96;  // lane-stack entries: 96 * 64 lanes * 8 B = 48 KB of LDS per block
-constexpr int kWavePassBlocks = 1024;  // persistent grid of the wave-per-query pass (4 per CU)
+constexpr int kWavePassBlocks = 2048;  // persistent grid of the wave-per-query passes (8 wavefronts per CU)
 // Packet steps per wavefront before its queries are deferred to the wave-per-query pass, by query
 // kind and dimension (measured optimum for 4-primitive leaves; a scene option overrides it).
 constexpr int kPacketBudgetCp3 = 1500, kPacketBudgetCp2 = 1000, kPacketBudgetSil = 1000;
@@ -495,8 +495,8 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
 // the frontier stays sorted by depth; every append at depth d happens after all older depth-d
 // entries were taken, hence at most 256 entries per depth level and 256 * (levels) in total.
 // Leaf candidates are reduced across the wave each round.
-constexpr int kWaveBlock = 128;
-constexpr int kFront = 2560;
+constexpr int kWaveBlock = 64;      // one wavefront per block: the frontier is sized per launch
+constexpr int kFrontMax = 8192;     // frontier entries at most (64 KB of LDS)
 
 __device__ __forceinline__ int wave_exclusive_scan(int v, int &total) {
     const int lane = threadIdx.x & 63;
@@ -511,7 +511,7 @@ __device__ __forceinline__ int wave_exclusive_scan(int v, int &total) {
 }
 
 template <int D, bool Q>
-__device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uint2 *front) {
+__device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uint2 *front, int cap) {
     using V = typename VT<D>::T;
     const int lane = threadIdx.x & 63;
     {
@@ -598,10 +598,10 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
             tot = (int)uniform((uint32_t)tot);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                if (hit[j] && pos < kFront) front[pos] = ent[j];
+                if (hit[j] && pos < cap) front[pos] = ent[j];
                 pos += (int)hit[j];
             }
-            size = size + tot < kFront ? size + tot : kFront;
+            size = size + tot < cap ? size + tot : cap;
         }
         cp_record<D>(P, qi, p, best_slot);
     }
@@ -1083,10 +1083,10 @@ __device__ __forceinline__ void sil_lane(const GP &P, int base) {
 // closest silhouette, wave per query: finishes the queries the packet pass deferred, with the same
 // node tests (reference box and cone test, exact cone cull).  Frontier as in the closest-point
 // pass: at most 128 entries per binary level.
-constexpr int kFrontSil = 3072;
+
 
 template <int D, bool Q>
-__device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, uint2 *front) {
+__device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, uint2 *front, int cap) {
     using V = typename VT<D>::T;
     const int lane = threadIdx.x & 63;
     {
@@ -1179,10 +1179,10 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
             const int nh = (int)h0 + (int)h1;
             int pos = size + wave_exclusive_scan(nh, tot);
             tot = (int)uniform((uint32_t)tot);
-            if (h0 && pos < kFrontSil) front[pos] = ch0;
+            if (h0 && pos < cap) front[pos] = ch0;
             pos += (int)h0;
-            if (h1 && pos < kFrontSil) front[pos] = ch1;
-            size = size + tot < kFrontSil ? size + tot : kFrontSil;
+            if (h1 && pos < cap) front[pos] = ch1;
+            size = size + tot < cap ? size + tot : cap;
         }
         sil_record<D>(P, qi, x, best_slot);
     }
@@ -1445,9 +1445,9 @@ __global__ void __launch_bounds__(kRayBlock) k_sil_lane(const EngineParams *J, c
 // 0) and all lanes store the same record: no lane-dependent branch inside the loop, so the
 // compiler cannot split the wavefront across iterations.
 template <int D, bool Q>
-__global__ void __launch_bounds__(kWaveBlock) k_cp_wave(const EngineParams *J, uint32_t *dc, const uint4 *deferred) {
-    __shared__ uint2 front_all[kWaveBlock / 64][kFront];
-    uint2 *front = front_all[threadIdx.x >> 6];
+__global__ void __launch_bounds__(kWaveBlock) k_cp_wave(const EngineParams *J, uint32_t *dc, const uint4 *deferred,
+                                                       int cap) {
+    extern __shared__ uint2 front[];   // cap entries
     const int lane = threadIdx.x & 63;
     const uint32_t total = uniform(dc[0]);
     for (;;) {
@@ -1455,14 +1455,14 @@ __global__ void __launch_bounds__(kWaveBlock) k_cp_wave(const EngineParams *J, u
         if (k >= total) break;
         const uint4 item = deferred[k];
         const GP P = gview(J[uniform(item.w)]);
-        cp_wave_item<D, Q>(P, item, front);
+        cp_wave_item<D, Q>(P, item, front, cap);
     }
 }
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kWaveBlock) k_sil_wave(const EngineParams *J, uint32_t *dc, const uint4 *deferred) {
-    __shared__ uint2 front_all[kWaveBlock / 64][kFrontSil];
-    uint2 *front = front_all[threadIdx.x >> 6];
+__global__ void __launch_bounds__(kWaveBlock) k_sil_wave(const EngineParams *J, uint32_t *dc, const uint4 *deferred,
+                                                        int cap) {
+    extern __shared__ uint2 front[];   // cap entries
     const int lane = threadIdx.x & 63;
     const uint32_t total = uniform(dc[0]);
     for (;;) {
@@ -1470,7 +1470,7 @@ __global__ void __launch_bounds__(kWaveBlock) k_sil_wave(const EngineParams *J, 
         if (k >= total) break;
         const uint4 item = deferred[k];
         const GP P = gview(J[uniform(item.w)]);
-        sil_wave_item<D, Q>(P, item, front);
+        sil_wave_item<D, Q>(P, item, front, cap);
     }
 }
 
@@ -1489,6 +1489,7 @@ struct Group {
     size_t dc_off = 0, def_off = 0;
     bool defer = false;
     int max_stack = 0;
+    int front = 0;                    // frontier entries of the wave-per-query pass
 };
 
 struct Plan {
@@ -1612,14 +1613,16 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
                 G.max_stack = std::max(G.max_stack, P.stack_depth);
                 continue;
             }
-            const bool fits = G.kind == kClosestPoint ? 256 * (w.wide_depth + 1) <= kFront
-                                                      : 128 * (w.bin_depth + 2) <= kFrontSil;
+            // frontier bound: 256 entries per BVH4 level, 128 per binary SNCH level
+            const int need = G.kind == kClosestPoint ? 256 * (w.wide_depth + 1) : 128 * (w.bin_depth + 2);
+            const bool fits = need <= kFrontMax;
             const int budget = jobs[j].budget != 0 ? jobs[j].budget : w.packet_budget;
             if (fits && budget >= 0) {
                 P.budget = budget > 0 ? budget
                          : G.kind == kSilhouette ? kPacketBudgetSil
                          : G.dim == 3 ? kPacketBudgetCp3 : kPacketBudgetCp2;
                 G.defer = true;
+                G.front = std::max(G.front, need);
                 nq += (size_t)jobs[j].q.n;
             }
         }
@@ -1655,11 +1658,15 @@ void launch_group_t(const Group &G, const EngineParams *J, const uint2 *tiles, u
     } else if (G.kind == kSilhouette) {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 2);
         hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt);
-        if (dc) hipLaunchKernelGGL((k_sil_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, J, dc, def);
+        if (dc)
+            hipLaunchKernelGGL((k_sil_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock),
+                               (size_t)G.front * sizeof(uint2), st, J, dc, def, G.front);
     } else {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 2);
         hipLaunchKernelGGL((k_cp_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt);
-        if (dc) hipLaunchKernelGGL((k_cp_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock), 0, st, J, dc, def);
+        if (dc)
+            hipLaunchKernelGGL((k_cp_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock),
+                               (size_t)G.front * sizeof(uint2), st, J, dc, def, G.front);
     }
 }
 

@@ -175,3 +175,29 @@ def scene_geometry(spec):
     if spec["dim"] == 3:
         return displaced_icosphere(spec["level"], seed=spec["geometry_seed"])
     return star_polyline(spec["segments"], seed=spec["geometry_seed"])
+
+
+def served_workload(n_scenes: int = 16, n_requests: int = 400, scale: float = 1.0, seed: int = WORKLOAD_SEED + 1):
+    """Resident scenes and request stream of the serving workload: scenes mixed 2D / 3D with
+    primitive counts log-uniform in [10k, 2M]; requests name a scene uniformly, a kind with
+    probabilities closest point 0.6 / ray 0.25 / silhouette 0.15, and a batch size log-uniform in
+    [256, 262144].  Returns (scene specs, request specs)."""
+    rng = _rng(seed)
+    scenes = []
+    for k in range(n_scenes):
+        target = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6)))) * scale
+        if k % 2:
+            levels = np.arange(1, 9)
+            s = int(levels[np.argmin(np.abs(np.log(20.0 * 4.0 ** levels) - np.log(max(target, 20.0))))])
+            spec = dict(dim=3, level=s, prims=20 * 4 ** s)
+        else:
+            spec = dict(dim=2, segments=max(int(round(target)), 3), prims=max(int(round(target)), 3))
+        spec.update(geometry_seed=GEOMETRY_SEED + 1000 * (k + 101))
+        scenes.append(spec)
+    kinds = rng.choice(["cp", "ray", "sil"], size=n_requests, p=[0.6, 0.25, 0.15])
+    reqs = []
+    for i in range(n_requests):
+        n = max(int(np.exp(rng.uniform(np.log(256), np.log(262144))) * scale), 1)
+        reqs.append(dict(scene=int(rng.integers(0, n_scenes)), kind=str(kinds[i]), n=n,
+                         query_seed=QUERY_SEED + 1000 * (i + 101)))
+    return scenes, reqs
