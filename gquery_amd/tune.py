@@ -1,0 +1,108 @@
+"""Autotuner: picks the engine's per-scene settings by timing candidates on the device.
+
+Knobs (all exact: every candidate returns the same records):
+  * leaf_prims     -- primitives per BVH4 leaf (node width of the leaf level: 2, 3, 4 or 6);
+  * packet_budget  -- packet steps a wavefront takes before its queries move to the
+                      wave-per-query pass (how much work runs wave-per-query vs 64-lane packets);
+  * traversal      -- silhouettes: packet + wave-per-query pass, or lane per query.
+The traversal stack depth is not a knob: it is fixed by the tree (the engine sizes the LDS stacks
+and frontiers from the scene's depth at launch).
+
+    best = tune_shape(Mesh, V, F, queries={"cp": x_cp, "sil": x_sil})
+    mesh = Mesh(V, F, **best["options"])        # leaf_prims, packet_budget, traversal
+
+Timing uses a sample of the caller's queries (or uniform points in the scene box enlarged by
+25%), median of a few runs per candidate after a warm-up.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+import numpy as np
+import torch
+
+LEAVES = (2, 3, 4, 6)
+BUDGETS = (750, 1000, 1500, 2000, 3000)
+TRAVERSALS = (1, 2)
+
+
+def _time(fn, reps=3):
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def _run(shape, kind, x, v):
+    if kind == "cp":
+        return shape.closest_point_bvh(x)
+    if kind == "ray":
+        return shape.intersect_bvh(x, v)
+    return shape.closest_silhouette_snch(x)
+
+
+def sample_queries(V, n, seed=0):
+    """Uniform points in the scene's box enlarged by 25% (and unit directions)."""
+    V = np.asarray(V, np.float32)
+    lo, hi = V.min(0), V.max(0)
+    c, h = 0.5 * (lo + hi), 0.625 * (hi - lo)
+    rng = np.random.default_rng(seed)
+    x = (c + rng.uniform(-1.0, 1.0, size=(n, V.shape[1])) * h).astype(np.float32)
+    d = rng.standard_normal(size=(n, V.shape[1])).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return x, d
+
+
+def tune_shape(cls, V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1 << 18,
+               device="cuda:0", leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, log=None):
+    """Returns {"options": {...}, "ms": {kind: ms}, "trials": [...]} for a Mesh / Polyline class.
+
+    queries: {kind: (x, v)} device tensors to time on (a sample of the real workload); otherwise
+    uniform points.  Cost: len(leaves) scene builds and len(leaves)*(len(budgets)+len(traversals))
+    timed calls per kind on n_sample queries."""
+    if queries is None:
+        x, v = sample_queries(V, n_sample)
+        x, v = torch.from_numpy(x).to(device), torch.from_numpy(v).to(device)
+        queries = {k: (x, v) for k in kinds}
+    trials = []
+    for leaf in leaves:
+        for budget, trav in itertools.product(budgets, traversals):
+            if trav == 2 and budget != budgets[0]:
+                continue        # lane per query ignores the budget: time it once
+            shape = cls(V, F, device=device, leaf_prims=leaf, packet_budget=budget, traversal=trav)
+            ms = {}
+            for k in kinds:
+                if k != "sil" and trav == 2:
+                    continue
+                x, v = queries[k]
+                ms[k] = _time(lambda: _run(shape, k, x, v))
+            trials.append({"leaf_prims": leaf, "packet_budget": budget, "traversal": trav, "ms": ms})
+            if log:
+                log(f"tune leaf={leaf} budget={budget} traversal={trav}: {ms}")
+            del shape
+    # leaf size and budget: minimise the summed closest-point + ray time; silhouettes pick their
+    # traversal (and budget) on the chosen leaf size
+    fast = [k for k in kinds if k != "sil"]
+    packet = [t for t in trials if t["traversal"] == 1]
+    if fast:
+        best = min(packet, key=lambda t: sum(t["ms"][k] for k in fast))
+    else:
+        best = min(packet, key=lambda t: t["ms"].get("sil", 0.0))
+    opts = {"leaf_prims": best["leaf_prims"], "packet_budget": best["packet_budget"], "traversal": 1}
+    ms = {k: best["ms"][k] for k in fast}
+    if "sil" in kinds:
+        same_leaf = [t for t in trials if t["leaf_prims"] == best["leaf_prims"] and "sil" in t["ms"]]
+        s = min(same_leaf, key=lambda t: t["ms"]["sil"])
+        opts["traversal"] = s["traversal"]
+        ms["sil"] = s["ms"]["sil"]
+        if not fast:
+            opts["packet_budget"] = s["packet_budget"]
+    return {"options": opts, "ms": ms, "trials": trials}
