@@ -4,7 +4,9 @@ Workload (the headline): displaced icosphere, 7 subdivisions (163,842 vertices, 
 triangles), 1,048,576 query points uniform in [-1.5, 1.5]^3, all resident in HBM before timing.
 A step is one Mesh.closest_point_bvh call over the whole batch.  The CPU baseline is the FP32
 restatement of the reference traversal (oracle/gq_oracle.c) on the host cores, over a bounded
-sample of the same queries.
+sample of the same queries.  The same run times the plain FP32 kernels on the same batch
+(`vs_plain`) and the serving loop (16 resident scenes, Poisson requests at 25/50/90% of the
+saturated rate: p50/p99 latency under `served`).
 
     python bench.py [--steps K] [--warmup W] [--method engine|engine_fp32|plain]
 """
@@ -21,6 +23,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+PLAIN_STEPS = 3
 N_QUERIES = 1 << 20
 SUBDIV = 7
 
@@ -96,7 +99,7 @@ def pmc_summary(args):
     trav = [k for k in fetch if k.startswith("void gqa::") or "gqa::" in k]
     rows, tot_f, tot_w = [], 0.0, 0.0
     for k in sorted(trav):
-        if not any(t in k for t in ("k_closest_point", "k_intersect", "k_silhouette", "k_zero")):
+        if not any(t in k for t in ("k_cp_", "k_ray_", "k_sil_", "k_zero")):
             continue
         fb = fetch[k].get("FETCH_SIZE", 0.0) * 1024.0 * f_scale
         wb = write.get(k, {}).get("WRITE_SIZE", 0.0) * 1024.0 * w_scale
@@ -107,6 +110,15 @@ def pmc_summary(args):
     res = {"hbm_bytes_per_launch": tot_f + tot_w, "fetch_bytes": tot_f, "write_bytes": tot_w,
            "fetch_scale": f_scale, "write_scale": w_scale, "copy_kernel": ck[:120],
            "kernels": {k[:120]: {"fetch_bytes": fb, "write_bytes": wb} for k, fb, wb in rows}}
+    if len(args.pmc_summary) > 2:
+        # L2 (TCC) hit rate per kernel: hits / (hits + misses) over all L2 requests of a dispatch
+        l2 = _pmc_table(args.pmc_summary[2])
+        hits = misses = 0.0
+        for k, _, _ in rows:
+            h, m = l2.get(k, {}).get("TCC_HIT_sum", 0.0), l2.get(k, {}).get("TCC_MISS_sum", 0.0)
+            res["kernels"][k[:120]]["l2_hit_rate"] = h / (h + m) if h + m else None
+            hits, misses = hits + h, misses + m
+        res["l2_hit_rate"] = hits / (hits + misses) if hits + misses else None
     (ROOT / "profiles" / f"{tag}.json").write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res, indent=1))
 
@@ -164,13 +176,21 @@ def accuracy(args):
                 # below it, where float32 coordinates carry absolute (not relative) error
                 rel64 = np.abs(gd - bd) / np.maximum(np.abs(np.nan_to_num(bd)), 1.0)
                 rel32 = np.abs(gd - ref["d"]) / np.maximum(np.abs(ref["d"]), 1.0)
+                rv = gv & ref["valid"].astype(bool)
                 out = {"scene": name, "query": kind, "method": method, "n": n,
                        "valid_vs_fp64": float((gv == bvalid).mean()),
                        "id_vs_fp64_where_gap": float(same.sum() / max((both & gap).sum(), 1)),
                        "max_err_d_vs_fp64": float(rel64[same].max()) if same.any() else 0.0,
+                       "mean_err_d_vs_fp64": float(rel64[same].mean()) if same.any() else 0.0,
                        "p99_err_d_vs_fp64": float(np.percentile(rel64[same], 99)) if same.any() else 0.0,
-                       "max_err_d_vs_fp32_restatement": float(rel32[gv & ref["valid"].astype(bool)].max())
-                       if (gv & ref["valid"].astype(bool)).any() else 0.0}
+                       "max_err_d_vs_fp32_restatement": float(rel32[rv].max()) if rv.any() else 0.0,
+                       "mean_err_d_vs_fp32_restatement": float(rel32[rv].mean()) if rv.any() else 0.0}
+                if "p" in bf:
+                    # point error: Euclidean distance to the FP64 point, over max(d, 1) like d
+                    gp = r.p.cpu().numpy().astype(np.float64).reshape(n, dim)
+                    pe = np.linalg.norm(gp - bf["p"], axis=1) / np.maximum(np.abs(np.nan_to_num(bd)), 1.0)
+                    out["max_err_p_vs_fp64"] = float(pe[same].max()) if same.any() else 0.0
+                    out["mean_err_p_vs_fp64"] = float(pe[same].mean()) if same.any() else 0.0
                 print(json.dumps(out), flush=True)
 
 
@@ -340,6 +360,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--method", default="engine", choices=["engine", "engine_fp32", "plain"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP32 kernels")
+    ap.add_argument("--no-served", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--leaf-prims", type=int, default=0, help="engine BVH4 leaf size (0 = default)")
     ap.add_argument("--traversal", type=int, default=0, help="silhouettes: 0 auto, 1 packet, 2 lane per query")
     ap.add_argument("--packet-budget", type=int, default=0,
@@ -354,9 +376,10 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after timing, run 3 elementwise float copies of this many MiB (the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE counter runs)")
-    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
                     help="no GPU: turn two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; csv) of "
-                         "`bench.py --copy-mb M` into profiles/traffic_<workload>_<method>.{json,md}")
+                         "`bench.py --copy-mb M`, and optionally a third (TCC_HIT_sum TCC_MISS_sum), "
+                         "into profiles/traffic_<workload>_<method>.json")
     ap.add_argument("--accuracy", type=int, default=0, metavar="N",
                     help="instead of timing: error of every method and query type against the FP64 "
                          "brute force and the FP32 restatement on N sampled queries")
@@ -504,8 +527,30 @@ def main():
         cpu = cpu_baseline(mesh, x_host)
         log(f"cpu baseline: {cpu['value']:.3f} Mq/s ({cpu['sample']})")
 
+    plain = None
+    if not args.no_plain and method != "plain":
+        # the plain FP32 kernels (reference layout, lane per query) on the same resident batch
+        mesh.closest_point_bvh(x, method="plain")
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(PLAIN_STEPS):
+            mesh.closest_point_bvh(x, method="plain")
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        p_ms = ev0.elapsed_time(ev1) / PLAIN_STEPS
+        plain = {"ms_per_step": p_ms, "value": nq / (p_ms * 1e-3) / 1e6, "steps": PLAIN_STEPS,
+                 "speedup": p_ms / ms_per_step}
+        log(f"plain FP32 kernels: {p_ms:.2f} ms/step, {plain['value']:.1f} Mq/s; engine speedup {plain['speedup']:.2f}x")
+
+    srv = None
+    if not args.no_served:
+        del rec
+        s = served(args)
+        srv = {"p99_ms_at_90pct_load": s["value"], "workload": s["config"]["workload"],
+               **{k: {a: round(b, 3) for a, b in v.items()} for k, v in s["served"].items()}}
+
     out = {
-        "metric": "Mqueries/s (3D closest-point, 327k tris, 1M q)",
+        "metric": "Mqueries/s (3D closest-point, 327k tris, 1M q) vs plain; served p99 latency",
         "value": value,
         "unit": "Mqueries/s",
         "n_gpus": 1,
@@ -522,6 +567,9 @@ def main():
                    "method": method, "global_batch": N_QUERIES, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "vs_plain": plain["speedup"] if plain else None,
+        "plain": plain,
+        "served": srv,
         "max_rel_err_vs_oracle": max_err,
     }
     print(json.dumps(out))

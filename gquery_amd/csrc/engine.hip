@@ -34,6 +34,7 @@ constexpr int kBlock = 256;     // packet kernels: 4 wavefronts per block
 constexpr int kRayBlock = 64;   // lane-per-query ray kernel: one wavefront per block
 constexpr int kRayStackMax = 96;  // lane-stack entries: 96 * 64 lanes * 8 B = 48 KB of LDS per block
 constexpr int kWavePassBlocks = 2048;  // persistent grid of the wave-per-query passes (8 wavefronts per CU)
+constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its own L2
 // Packet steps per wavefront before its queries are deferred to the wave-per-query pass, by query
 // kind and dimension (measured optimum for 4-primitive leaves; a scene option overrides it).
 constexpr int kPacketBudgetCp3 = 1500, kPacketBudgetCp2 = 1000, kPacketBudgetSil = 1000;
@@ -1217,6 +1218,17 @@ __device__ __forceinline__ uint2 tile_at(const uint2 *tiles, int t, int ts) {
     return tiles ? tiles[t] : make_uint2(0u, (uint32_t)t * (uint32_t)ts);
 }
 
+// Workgroup b of a launch of n goes to XCD b % kXcds (the dispatcher deals workgroups round robin
+// over the XCDs).  With spread on, XCD x takes one contiguous run of tiles instead of every 8th
+// tile, so on Morton-sorted queries each XCD's L2 sees one region of space and its part of the
+// scene.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_tile(int b, int n, int spread) {
+    if (!spread) return b;
+    const int x = b % kXcds, local = b / kXcds;
+    const int per = n / kXcds, rem = n % kXcds;
+    return x * per + min(x, rem) + local;
+}
+
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
     v &= 0x3FFu;
     v = (v | (v << 16)) & 0x030000FFu;
@@ -1414,29 +1426,29 @@ __global__ void __launch_bounds__(kBlock) k_sort_refine(const EngineParams *J) {
 // tiles; the wave-per-query passes drain the deferred list of their launch group, whose entries
 // name their job.
 template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_cp_packet(const EngineParams *J, const uint2 *tiles) {
-    const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
+__global__ void __launch_bounds__(kBlock) k_cp_packet(const EngineParams *J, const uint2 *tiles, int spread) {
+    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kBlock);
     const GP P = gview(J[tl.x]);
     cp_packet<D, Q>(P, (int)tl.y);
 }
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_sil_packet(const EngineParams *J, const uint2 *tiles) {
-    const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
+__global__ void __launch_bounds__(kBlock) k_sil_packet(const EngineParams *J, const uint2 *tiles, int spread) {
+    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kBlock);
     const GP P = gview(J[tl.x]);
     sil_packet<D, Q>(P, (int)tl.y);
 }
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kRayBlock) k_ray_lane(const EngineParams *J, const uint2 *tiles) {
-    const uint2 tl = tile_at(tiles, blockIdx.x, kRayBlock);
+__global__ void __launch_bounds__(kRayBlock) k_ray_lane(const EngineParams *J, const uint2 *tiles, int spread) {
+    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kRayBlock);
     const GP P = gview(J[tl.x]);
     ray_lane<D, Q>(P, (int)tl.y);
 }
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kRayBlock) k_sil_lane(const EngineParams *J, const uint2 *tiles) {
-    const uint2 tl = tile_at(tiles, blockIdx.x, kRayBlock);
+__global__ void __launch_bounds__(kRayBlock) k_sil_lane(const EngineParams *J, const uint2 *tiles, int spread) {
+    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kRayBlock);
     const GP P = gview(J[tl.x]);
     sil_lane<D, Q>(P, (int)tl.y);
 }
@@ -1644,6 +1656,15 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
     return hipSuccess;
 }
 
+// XCD-contiguous tile order for the traversal launches (env GQ_XCD_SPREAD=0/1 overrides).
+int xcd_spread() {
+    static const int v = [] {
+        const char *e = getenv("GQ_XCD_SPREAD");
+        return e ? atoi(e) : 0;
+    }();
+    return v;
+}
+
 template <int D, bool Q>
 void launch_group_t(const Group &G, const EngineParams *J, const uint2 *tiles, uint8_t *base, hipStream_t st) {
     const uint2 *gt = tiles ? tiles + G.tiles_off : nullptr;   // null: implicit tiles of job 0
@@ -1652,18 +1673,18 @@ void launch_group_t(const Group &G, const EngineParams *J, const uint2 *tiles, u
     uint32_t *dc = G.defer ? (uint32_t *)(base + G.dc_off) : nullptr;
     const uint4 *def = G.defer ? (const uint4 *)(base + G.def_off) : nullptr;
     if (G.kind == kIntersect) {
-        hipLaunchKernelGGL((k_ray_lane<D, Q>), dim3(nt), dim3(kRayBlock), (size_t)G.max_stack * kRayBlock * sizeof(uint2), st, J, gt);
+        hipLaunchKernelGGL((k_ray_lane<D, Q>), dim3(nt), dim3(kRayBlock), (size_t)G.max_stack * kRayBlock * sizeof(uint2), st, J, gt, xcd_spread());
     } else if (G.kind == kSilhouette && G.lane) {
-        hipLaunchKernelGGL((k_sil_lane<D, Q>), dim3(nt), dim3(kRayBlock), (size_t)G.max_stack * kRayBlock * sizeof(uint2), st, J, gt);
+        hipLaunchKernelGGL((k_sil_lane<D, Q>), dim3(nt), dim3(kRayBlock), (size_t)G.max_stack * kRayBlock * sizeof(uint2), st, J, gt, xcd_spread());
     } else if (G.kind == kSilhouette) {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 2);
-        hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt);
+        hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt, xcd_spread());
         if (dc)
             hipLaunchKernelGGL((k_sil_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock),
                                (size_t)G.front * sizeof(uint2), st, J, dc, def, G.front);
     } else {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 2);
-        hipLaunchKernelGGL((k_cp_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt);
+        hipLaunchKernelGGL((k_cp_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt, xcd_spread());
         if (dc)
             hipLaunchKernelGGL((k_cp_wave<D, Q>), dim3(kWavePassBlocks), dim3(kWaveBlock),
                                (size_t)G.front * sizeof(uint2), st, J, dc, def, G.front);
