@@ -172,14 +172,17 @@ def accuracy(args):
                 gid = r.prim_id.cpu().numpy()
                 both = gv & bvalid
                 same = both & gap & (gid == bf["prim_id"])
+                rv = gv & ref["valid"].astype(bool)
                 # error relative to max(d, 1): relative beyond the scene scale (radius ~1), absolute
                 # below it, where float32 coordinates carry absolute (not relative) error
                 rel64 = np.abs(gd - bd) / np.maximum(np.abs(np.nan_to_num(bd)), 1.0)
                 rel32 = np.abs(gd - ref["d"]) / np.maximum(np.abs(ref["d"]), 1.0)
-                rv = gv & ref["valid"].astype(bool)
                 out = {"scene": name, "query": kind, "method": method, "n": n,
                        "valid_vs_fp64": float((gv == bvalid).mean()),
                        "id_vs_fp64_where_gap": float(same.sum() / max((both & gap).sum(), 1)),
+                       "id_vs_fp32_restatement_where_gap": float((rv & gap & (gid == ref["prim_id"])).sum()
+                                                                 / max((rv & gap).sum(), 1)),
+                       "n_gap": int((both & gap).sum()),
                        "max_err_d_vs_fp64": float(rel64[same].max()) if same.any() else 0.0,
                        "mean_err_d_vs_fp64": float(rel64[same].mean()) if same.any() else 0.0,
                        "p99_err_d_vs_fp64": float(np.percentile(rel64[same], 99)) if same.any() else 0.0,

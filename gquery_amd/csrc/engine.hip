@@ -1520,6 +1520,12 @@ int cell_bits_for(int64_t n, int dim) {
     return std::max(b, 2);
 }
 
+// Whether a job's queries are Morton-sorted before traversal.  Rays are not: origins alone do not
+// make rays coherent (directions stay random), and on 1M uniform rays the traversal kernel is as
+// fast in input order (3D 0.70 vs 0.69 ms, 2D 0.29 vs 0.33 ms) while the sort costs 0.2 ms.
+// Closest-point and silhouette queries gain 1.2x (3D silhouettes) to 18x (2D closest point).
+bool sort_job(int kind, int64_t n) { return n >= kSortMin && kind != kIntersect; }
+
 // Fills the plan for `jobs` with every device pointer relative to `base` (null when only the
 // sizes are wanted).
 hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &plan) {
@@ -1553,7 +1559,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
     for (int di = 0; di < 2; ++di) {
         plan.sort_tiles_off[di] = plan.tiles.size();
         for (size_t j = 0; j < nj; ++j) {
-            if (jobs[j].w->dim != di + 2 || jobs[j].q.n < kSortMin) continue;
+            if (jobs[j].w->dim != di + 2 || !sort_job(jobs[j].kind, jobs[j].q.n)) continue;
             for (int64_t t = 0; t < jobs[j].q.n; t += kBlock) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
         }
         plan.n_sort_tiles[di] = plan.tiles.size() - plan.sort_tiles_off[di];
@@ -1602,7 +1608,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             P.ids = w.prim_ids;
             P.root = w.root_ref;
         }
-        if (jb.q.n >= kSortMin) {
+        if (sort_job(jb.kind, jb.q.n)) {
             P.cell_bits = cell_bits_for(jb.q.n, w.dim);
             P.n_cells = 1 << (P.cell_bits * w.dim);
             P.cell_shift = w.dim * ((w.dim == 3 ? 10 : 16) - P.cell_bits);
@@ -1769,7 +1775,7 @@ hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, cons
     // A single call sorts with a device radix sort of the full Morton codes (fastest for one
     // large batch); batches use the counting sort, which records into graphs.
     const int n = (int)q.n;
-    const bool sorted = n >= kSortMin;
+    const bool sorted = sort_job(kind, n);
     size_t temp = 0;
     if (sorted) {
         e = rocprim::radix_sort_pairs(nullptr, temp, (uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,

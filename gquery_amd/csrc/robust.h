@@ -1,13 +1,14 @@
 // FP64 re-evaluation of the float32 decisions that rounding can flip (engine kernels only; the
 // plain kernels keep the reference's float32 arithmetic throughout).
 //
-//  * Möller–Trumbore (math.py:110-128, triangle.py:48-73) when the determinant is small next to
-//    the edge and direction lengths: a grazing ray, where float32 t, v and w lose most digits.
+//  * Möller–Trumbore (math.py:110-128, triangle.py:48-73) when the determinant is below 0.05 of
+//    the product of the edge and direction lengths: a grazing ray, where float32 t, v and w lose
+//    digits.
 //  * The silhouette-edge sign predicate (silhouette_edge.py:7-12) and the silhouette-vertex sign
 //    predicate (silhouette_vertex.py:26-37) when either factor is within its float32 rounding
 //    bound of zero.
 // Outside those bands the float32 result is returned unchanged, so the engine agrees with the
-// reference bit for bit there and with an FP64 evaluation of the same formulas everywhere.
+// reference bit for bit there; inside them it returns the FP64 evaluation of the same formulas.
 #pragma once
 
 #include "gq_device.h"
@@ -30,12 +31,15 @@ __device__ __forceinline__ float ray_triangle_robust(V3 x, V3 d, V3 a, V3 b, V3 
     const V3 v1 = b - a, v2_ = c - a;
     const V3 p = cross(d, v2_);
     const float det = dot(v1, p);
-    // |det| = |v1|·|d|·|v2|·|sin| terms; below 1e-4 of that scale float32 has < 3 good digits left
+    // |det| = |v1|·|d|·|v2|·|sin| terms.  The float32 t carries a relative error of about
+    // 2e-8 · scale / |det| (measured against FP64 on grazing rays over a flat grid: 1.6e-5 at
+    // |det| = 1e-3 scale, 6.8e-7 at 0.03-0.1 scale), so below 0.05 of the scale the test runs in
+    // FP64 and the float32 path keeps t within ~4e-7 of the FP64 value.
     const float scale = (fabsf(v1.x) + fabsf(v1.y) + fabsf(v1.z)) * (fabsf(d.x) + fabsf(d.y) + fabsf(d.z)) *
                         (fabsf(v2_.x) + fabsf(v2_.y) + fabsf(v2_.z));
     vv = 0.0f;
     ww = 0.0f;
-    if (fabsf(det) > 1e-4f * scale) {
+    if (fabsf(det) > 0.05f * scale) {
         if (!(fabsf(det) > kDrEps)) return __builtin_inff();
         const float inv_det = 1.0f / det;
         const V3 s = x - a;

@@ -94,13 +94,67 @@ def test_parity_vs_oracle(gpu, name, kind, method):
     got = _np(_query(shape, kind, method, x, v))
     gap = _gap_from_bruteforce(kind, dim, shape.snch, V, F, x, v, ref)
     vec = ("p", "n") if kind != "sil" else ("p",)
-    res = parity.summarize(f"{name}/{kind}/{method}", got, ref, gap, vec_fields=vec)
+    keep = np.ones(n, bool)
+    if kind == "ray" and method != "plain" and dim == 3:
+        # The engine re-evaluates grazing ray-triangle tests in FP64: there it must meet the FP64
+        # bound, and the FP32 reference is compared only where it is itself that accurate.
+        b = O.bf_intersect(dim, V, F, x, v)
+        hit = np.isfinite(b["t"]) & got["valid"].astype(bool)
+        assert parity.rel_err(got["d"][hit], b["t"][hit]).max(initial=0.0) <= parity.TOL_F64
+        with np.errstate(invalid="ignore"):
+            keep = ~np.isfinite(b["t"]) | (parity.rel_err(ref["d"], b["t"]) <= 0.1 * parity.TOL_REF)
+    sub = lambda r: {k: a[keep] for k, a in r.items() if isinstance(a, np.ndarray) and len(a) == n}
+    res = parity.summarize(f"{name}/{kind}/{method}", sub(got), sub(ref), gap[keep], vec_fields=vec)
     assert res["ok"], res
     if method == "plain" and kind != "sil":
         # same arithmetic as the oracle: bit-identical records
         assert np.array_equal(got["prim_id"], ref["prim_id"])
         assert np.array_equal(got["d"].view(np.int32), ref["d"].view(np.int32))
         assert np.array_equal(got["p"].view(np.int32), ref["p"].view(np.int32))
+
+
+def _grazing_rays(n, seed=5):
+    """A flat 8x8 grid of 128 triangles, rotated, and rays that meet it at grazing angles
+    (direction slope 2e-5 to 3e-4 against the plane): Moller-Trumbore determinants of 1e-4 to
+    1e-6 of their scale, where float32 t loses 2 to 5 digits."""
+    g = np.linspace(-1.0, 1.0, 9)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    V = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1)
+    F = []
+    for i in range(8):
+        for j in range(8):
+            a, b, c, d = i * 9 + j, (i + 1) * 9 + j, (i + 1) * 9 + j + 1, i * 9 + j + 1
+            F += [[a, b, c], [a, c, d]]
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    slope = np.exp(rng.uniform(np.log(2e-5), np.log(3e-4), n))
+    L = rng.uniform(0.1, 0.4, n)
+    o = np.stack([rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n), slope * L], 1)
+    d = np.stack([np.cos(phi), np.sin(phi), -slope], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return ((V @ Q.T).astype(np.float32), np.array(F, np.int32), (o @ Q.T).astype(np.float32),
+            (d @ Q.T).astype(np.float32))
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_grazing_rays_fp64_fallback(gpu, method):
+    from gquery_amd.shapes.mesh import Mesh
+    V, F, x, v = _grazing_rays(4096)
+    shape = Mesh(V, F, device="cuda:0")
+    got = _np(shape.intersect_bvh(x, v, method=method))
+    b = O.bf_intersect(3, V, F, x, v)
+    hit = np.isfinite(b["t"])
+    assert hit.all()
+    err = parity.rel_err(got["d"], b["t"])
+    if method == "plain":
+        # the reference's float32 arithmetic: most grazing hits are off by more than 1e-5
+        assert (err > parity.TOL_REF).mean() > 0.5
+        return
+    assert np.array_equal(got["valid"].astype(bool), hit)
+    assert err.max() <= parity.TOL_F64, err.max()
+    gap = parity.gap_mask(b["t"], b["second_t"])
+    assert np.array_equal(got["prim_id"][gap], b["prim_id"][gap])
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -216,7 +270,8 @@ def test_silhouette_traversal_modes(gpu, traversal, name, method):
 @pytest.mark.parametrize("kind", ["cp", "ray", "sil"])
 @pytest.mark.parametrize("method", ["engine_fp32", "engine"])
 def test_sorted_batch(gpu, name, kind, method):
-    """Batches of 8,192 queries and more go through the Morton-cell counting sort first."""
+    """Batches of 8,192 queries and more go through the Morton-cell counting sort first (closest
+    point and silhouettes; rays keep their input order)."""
     V, F, shape = scene(name)
     dim = V.shape[1]
     n = 20000
