@@ -730,17 +730,42 @@ __device__ __forceinline__ void ray_lane(const GP &P, int base) {
 
 // ---------------------------------------------------------------------------------------------
 // silhouette (packet)
+// One child of an SNCH pair.  3D: exact-cull cone in axis/cos_t/sin_t, reference cone (if it can
+// cull) in the table at ref_cone.  2D: the reference cone inline (axis[0..1], half, radius) and
+// cos_t = 2 for a subtree without silhouettes.
 struct PairChild {
     float lo[3], hi[3];
     float axis[3], cos_t, sin_t;
     uint32_t child;
     int32_t ref_cone;
+    float half, radius;
 };
 
 template <int D, bool Q>
 __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairChild (&c)[2]) {
-    if constexpr (Q) {
-        // SPairQ<D> (64 B): origin, exp | child boxes | cone_axis int16 @28 | cone_cs int16 @40 |
+    if constexpr (D == 2) {
+        // SPairF<2> (128 B, both node formats): lo[2][2] @0, hi[2][2] @16, cone axis @32, half @48,
+        // radius @56, child @64, ref_cone @72, cull @80
+        Words<24> r;
+        r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 128);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                c[j].lo[k] = r.f32(8 * j + 4 * k);
+                c[j].hi[k] = r.f32(16 + 8 * j + 4 * k);
+                c[j].axis[k] = r.f32(32 + 8 * j + 4 * k);
+            }
+            c[j].lo[2] = c[j].hi[2] = c[j].axis[2] = 0.0f;
+            c[j].half = r.f32(48 + 4 * j);
+            c[j].radius = r.f32(56 + 4 * j);
+            c[j].child = r.u32(64 + 4 * j);
+            c[j].ref_cone = (int32_t)r.u32(72 + 4 * j);
+            c[j].cos_t = r.u32(80 + 4 * j) ? 2.0f : 0.0f;
+            c[j].sin_t = 1.0f;
+        }
+    } else if constexpr (Q) {
+        // SPairQ<3> (64 B): origin, exp | child boxes | cone_axis int16 @28 | cone_cs int16 @40 |
         // child @48 | ref_cone @56
         Words<16> r;
         r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64);
@@ -825,6 +850,20 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
 }
 
 
+// The reference cone test (SNCHNode.overlaps, snch.py) of a child whose reference cone can cull,
+// evaluated with the reference's float32 box and cone: inline in the 2D pair record, from the
+// reference-cone table in 3D.  d_min receives the box distance.
+template <int D>
+__device__ __forceinline__ bool ref_cone_pass(const GP &P, const PairChild &c, const typename VT<D>::T &x, float r,
+                                              float &d_min) {
+    if constexpr (D == 2) {
+        return snch_overlaps(ldv<D>(c.lo), ldv<D>(c.hi), v2(c.axis[0], c.axis[1]), c.half, c.radius, x, r, d_min);
+    } else {
+        const RefCone rc = ldg(P.ref_cones + c.ref_cone);
+        return snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, d_min);
+    }
+}
+
 template <int D, bool Q>
 __device__ __forceinline__ void sil_packet(const GP &P, int base) {
     using V = typename VT<D>::T;
@@ -878,10 +917,7 @@ __device__ __forceinline__ void sil_packet(const GP &P, int base) {
                     float dm, dx;
                     box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                     bool h = act && dm <= r;
-                    if (h && c[j].ref_cone >= 0) {
-                        const RefCone rc = ldg(P.ref_cones + c[j].ref_cone);
-                        h = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dm);
-                    }
+                    if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
                     if (h && exact_cull<D>(c[j], xc)) h = false;
                     hh[j] = h;
                     hm[j] = __ballot(h);
@@ -1030,9 +1066,8 @@ __device__ __forceinline__ void sil_lane(const GP &P, int base) {
                 box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                 bool hj = dm <= r;
                 if (hj && c[j].ref_cone >= 0) {
-                    const RefCone rc = ldg(P.ref_cones + c[j].ref_cone);
                     float dr;
-                    hj = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dr);
+                    hj = ref_cone_pass<D>(P, c[j], x, r, dr);
                 }
                 if (hj && exact_cull<D>(c[j], xc)) hj = false;
                 h[j] = hj;
@@ -1122,10 +1157,7 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
                         float dm, dx;
                         box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                         bool h = dm <= r;
-                        if (h && c[j].ref_cone >= 0) {
-                            const RefCone rc = ldg(P.ref_cones + c[j].ref_cone);
-                            h = snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, dm);
-                        }
+                        if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
                         if (h && exact_cull<D>(c[j], xc)) h = false;
                         const uint2 ent = make_uint2(c[j].child, __float_as_uint(dm));
                         if (j == 0) { h0 = h; ch0 = ent; }

@@ -266,6 +266,7 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     if (h.is_snch) {
         if ((rc = upload(s.get(), wh.snodes.data(), wh.snodes.size(), (const uint8_t **)&wv.snodes))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.qsnodes.data(), wh.qsnodes.size(), (const uint8_t **)&wv.qsnodes))) return cleanup(rc);
+        if (h.dim == 2) wv.qsnodes = wv.snodes;   // 2D pairs have one (float32, cones inline) format
         if ((rc = upload(s.get(), wh.sils.data(), wh.sils.size(), &wv.sils))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.sil_ids.data(), wh.sil_ids.size(), &wv.sil_ids))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.ref_cones.data(), wh.ref_cones.size(), &wv.ref_cones))) return cleanup(rc);

@@ -330,9 +330,38 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
         if (!b[i].leaf) pidx[i] = np++;
     out.n_snch_nodes = np;
     out.snodes.assign((size_t)std::max<uint32_t>(np, 1) * sizeof(SPairF<D>), 0);
+    const float kHalfPi = 1.57079637050628662109375f;
+    if constexpr (D == 2) {
+        // one float32 record per pair with the reference boxes and cones inline (no quantized twin)
+        out.qsnodes.clear();
+        for (size_t i = 0; i < nn; ++i) {
+            if (b[i].leaf) continue;
+            const size_t ch[2] = {i + 1, i + b[i].second};
+            SPairF<2> pf;
+            std::memset(&pf, 0, sizeof pf);
+            for (int j = 0; j < 2; ++j) {
+                const size_t c = ch[j];
+                const float *n = snd + c * nss;   // SNCH node: box 0..3, axis 4..5, half 6, radius 7
+                for (int k = 0; k < 2; ++k) {
+                    pf.lo[j][k] = n[k];
+                    pf.hi[j][k] = n[2 + k];
+                    pf.cone_axis[j][k] = n[4 + k];
+                }
+                pf.cone_half[j] = n[6];
+                pf.cone_radius[j] = n[7];
+                pf.child[j] = child_ref(c, pidx);
+                // the reference cone test can only cull when 0 <= half <= pi/2 (a wider cone always
+                // contains a direction perpendicular to every view direction; a negative one is unset)
+                pf.ref_cone[j] = (n[6] >= 0.0f && !(n[6] > kHalfPi)) ? 0 : -1;
+                ConeF cf;
+                cone_of(c, cf);
+                pf.cull[j] = cf.cos_t > 1.5f ? 1 : 0;
+            }
+            std::memcpy(out.snodes.data() + (size_t)pidx[i] * sizeof(SPairF<2>), &pf, sizeof pf);
+        }
+    } else {
     out.qsnodes.assign((size_t)std::max<uint32_t>(np, 1) * sizeof(SPairQ<D>), 0);
     constexpr int L = QBits<D>::levels;
-    const float kHalfPi = 1.57079637050628662109375f;
     for (size_t i = 0; i < nn; ++i) {
         if (b[i].leaf) continue;
         const size_t ch[2] = {i + 1, i + b[i].second};
@@ -404,6 +433,7 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
         }
         std::memcpy(out.snodes.data() + (size_t)pidx[i] * sizeof(SPairF<D>), &pf, sizeof pf);
         std::memcpy(out.qsnodes.data() + (size_t)pidx[i] * sizeof(SPairQ<D>), &pq, sizeof pq);
+    }
     }
     out.sroot_ref = b[0].leaf ? lt.ref(sb[0], sc[0]) : 0u;
     for (int k = 0; k < D; ++k) { out.sroot_lo[k] = b[0].lo[k]; out.sroot_hi[k] = b[0].hi[k]; }

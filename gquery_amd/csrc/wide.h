@@ -94,9 +94,22 @@ template <int D> struct alignas(64) SPairF {
     int32_t ref_cone[2];    // index into the reference-cone table, -1 = the reference test cannot cull
     uint8_t pad[128 - (4 * D * 4 + 2 * 20 + 16)];
 };
+// 2D: the reference cone test can cull at nearly every node (its float32 half-angles are small),
+// and it needs the node's float32 box and cone exactly, so the 2D pair record carries both
+// children's reference boxes and cones inline: one 128-byte record per visit instead of a pair
+// record plus two reference-cone fetches.  2D silhouettes use this record for both node formats.
+template <> struct alignas(64) SPairF<2> {
+    float lo[2][2], hi[2][2];            // the children's boxes (SNCH node data, float32)
+    float cone_axis[2][2];               // reference cone axis of each child
+    float cone_half[2], cone_radius[2];  // reference half-angle and radius
+    uint32_t child[2];
+    int32_t ref_cone[2];                 // 0 = the reference cone test can cull, -1 = it cannot
+    int32_t cull[2];                     // 1 = no silhouette below this child (always skipped)
+    uint8_t pad[40];
+};
 static_assert(sizeof(SPairF<3>) == 128 && sizeof(SPairF<2>) == 128, "SPairF is two lines");
 
-// SNCH pair record, quantized boxes (8-bit 3D / 16-bit 2D) and int16 cones.
+// SNCH pair record (3D), 8-bit quantized boxes and int16 cones.
 template <int D> struct SPairQ;
 template <> struct alignas(64) SPairQ<3> {
     float origin[3];
@@ -108,26 +121,17 @@ template <> struct alignas(64) SPairQ<3> {
     uint32_t child[2];
     int32_t ref_cone[2];
 };
-template <> struct alignas(64) SPairQ<2> {
-    float origin[2];
-    int8_t exp[2];
-    uint8_t pad0[2];
-    uint16_t qlo[2][2], qhi[2][2];
-    int16_t cone_axis[2][3];
-    int16_t cone_cs[2][2];
-    uint32_t child[2];
-    int32_t ref_cone[2];
-};
-static_assert(sizeof(SPairQ<3>) == 64 && sizeof(SPairQ<2>) == 64, "SPairQ is one line");
+static_assert(sizeof(SPairQ<3>) == 64, "SPairQ is one line");
 // byte offsets the kernels decode (engine.hip load_pair / load_node4)
 static_assert(offsetof(SPairQ<3>, qlo) == 16 && offsetof(SPairQ<3>, qhi) == 22, "SPairQ<3> boxes");
-static_assert(offsetof(SPairQ<2>, qlo) == 12 && offsetof(SPairQ<2>, qhi) == 20, "SPairQ<2> boxes");
-static_assert(offsetof(SPairQ<3>, cone_axis) == 28 && offsetof(SPairQ<2>, cone_axis) == 28, "cone axis");
-static_assert(offsetof(SPairQ<3>, cone_cs) == 40 && offsetof(SPairQ<2>, cone_cs) == 40, "cone cs");
-static_assert(offsetof(SPairQ<3>, child) == 48 && offsetof(SPairQ<2>, child) == 48, "pair child");
-static_assert(offsetof(SPairQ<3>, ref_cone) == 56 && offsetof(SPairQ<2>, ref_cone) == 56, "pair ref");
+static_assert(offsetof(SPairQ<3>, cone_axis) == 28, "cone axis");
+static_assert(offsetof(SPairQ<3>, cone_cs) == 40, "cone cs");
+static_assert(offsetof(SPairQ<3>, child) == 48, "pair child");
+static_assert(offsetof(SPairQ<3>, ref_cone) == 56, "pair ref");
 static_assert(offsetof(SPairF<3>, cone) == 48 && offsetof(SPairF<3>, child) == 88 && offsetof(SPairF<3>, ref_cone) == 96, "SPairF<3>");
-static_assert(offsetof(SPairF<2>, cone) == 32 && offsetof(SPairF<2>, child) == 72 && offsetof(SPairF<2>, ref_cone) == 80, "SPairF<2>");
+static_assert(offsetof(SPairF<2>, cone_axis) == 32 && offsetof(SPairF<2>, cone_half) == 48 &&
+              offsetof(SPairF<2>, cone_radius) == 56 && offsetof(SPairF<2>, child) == 64 &&
+              offsetof(SPairF<2>, ref_cone) == 72 && offsetof(SPairF<2>, cull) == 80, "SPairF<2>");
 static_assert(offsetof(QNode<3>, qlo) == 16 && offsetof(QNode<3>, qhi) == 28 && offsetof(QNode<3>, child) == 40, "QNode<3>");
 static_assert(offsetof(QNode<2>, nchild) == 10 && offsetof(QNode<2>, qlo) == 12 && offsetof(QNode<2>, qhi) == 28 &&
               offsetof(QNode<2>, child) == 44, "QNode<2>");

@@ -161,10 +161,11 @@ int gq_batch_enqueue(int n_jobs, const gq_job *jobs, void *device_ws, int64_t de
 /* ---- Engine layout inspection (no counterpart in the reference: exposes the host-built engine
  * arrays so tests can check them on machines without a GPU) ---- */
 
-/* which: 0 quantized BVH4 nodes, 1 float32 BVH4 nodes, 2 quantized SNCH pairs, 3 float32 SNCH
- * pairs, 4 primitives (float4), 5 primitive ids, 6 silhouettes (float4), 7 silhouette ids,
- * 8 reference cones, 9 big-leaf table, 10 scalars (root_ref, sroot_ref, n_nodes, n_pairs,
- * wide_depth as uint32).  Pass out = NULL to get the byte count in *n_bytes. */
+/* which: 0 quantized BVH4 nodes, 1 float32 BVH4 nodes, 2 quantized SNCH pairs (3D only), 3 float32
+ * SNCH pairs (2D: with the reference cones inline), 4 primitives (float4), 5 primitive ids,
+ * 6 silhouettes (float4), 7 silhouette ids, 8 reference cones (3D only), 9 big-leaf table,
+ * 10 scalars (root_ref, sroot_ref, n_nodes, n_pairs, wide_depth as uint32).  Pass out = NULL to
+ * get the byte count in *n_bytes. */
 int gq_structure_engine_layout(const gq_structure *s, int which, void *out, int64_t *n_bytes);
 
 /* ---- Kernel timing (replaces the time.time() brackets around dr.eval in the reference demos,

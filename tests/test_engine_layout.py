@@ -23,16 +23,17 @@ def fnode_dtype(D):
 
 
 def spq_dtype(D):
-    if D == 3:
-        return np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("pad0", "u1"), ("qlo", "u1", (2, 3)),
-                         ("qhi", "u1", (2, 3)), ("axis", "<i2", (2, 3)), ("cs", "<i2", (2, 2)),
-                         ("child", "<u4", 2), ("ref", "<i4", 2)])
-    return np.dtype([("origin", "<f4", 2), ("exp", "i1", 2), ("pad0", "u1", 2), ("qlo", "<u2", (2, 2)),
-                     ("qhi", "<u2", (2, 2)), ("axis", "<i2", (2, 3)), ("cs", "<i2", (2, 2)),
+    assert D == 3   # 2D pairs have only the float32 record
+    return np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("pad0", "u1"), ("qlo", "u1", (2, 3)),
+                     ("qhi", "u1", (2, 3)), ("axis", "<i2", (2, 3)), ("cs", "<i2", (2, 2)),
                      ("child", "<u4", 2), ("ref", "<i4", 2)])
 
 
 def spf_dtype(D):
+    if D == 2:
+        return np.dtype([("lo", "<f4", (2, 2)), ("hi", "<f4", (2, 2)), ("cone_axis", "<f4", (2, 2)),
+                         ("half", "<f4", 2), ("radius", "<f4", 2), ("child", "<u4", 2), ("ref", "<i4", 2),
+                         ("cull", "<i4", 2), ("pad", "u1", 40)])
     return np.dtype([("lo", "<f4", (2, D)), ("hi", "<f4", (2, D)), ("axis0", "<f4", 3), ("cos0", "<f4"),
                      ("sin0", "<f4"), ("axis1", "<f4", 3), ("cos1", "<f4"), ("sin1", "<f4"),
                      ("child", "<u4", 2), ("ref", "<i4", 2), ("pad", "u1", 128 - (4 * D * 4 + 56))])
@@ -134,13 +135,21 @@ def test_snch_pair_layout(name):
     s = (ext.SNCH if D == 2 else ext.SNCH3D)(V, F)
     scal = s.engine_layout(10).view(np.uint32)
     sroot, n_pairs = scal[1], scal[3]
-    pq = s.engine_layout(2).view(spq_dtype(D))
     pf = s.engine_layout(3).view(spf_dtype(D))
+    if D == 3:
+        pq = s.engine_layout(2).view(spq_dtype(D))
+    else:
+        assert s.engine_layout(2).size == 0 and s.engine_layout(8).size == 0
+        pq = None
+        # SNCH node records of the reference (box, cone axis, half-angle, radius) in node order;
+        # children of pair records are matched to them through the subtree primitive ranges below
+        snd = s.node_data().reshape(-1, 13)
+        node_keys = {r.tobytes() for r in np.ascontiguousarray(snd[:, :8], np.float32)}
     sils = s.engine_layout(6).view(np.float32).reshape(-1, 12 if D == 3 else 8)
     sil_ids = s.engine_layout(7).view(np.int32)
     big = s.engine_layout(9).view(np.uint32)
     ref_sil = s.silhouette_data().reshape(-1, 19 if D == 3 else 10)
-    assert len(pq) == len(pf) == n_pairs and sroot == 0
+    assert len(pf) == n_pairs and sroot == 0 and (pq is None or len(pq) == n_pairs)
     np.testing.assert_array_equal(sil_ids, ref_sil[:, -1].astype(np.int32))
     if D == 3:
         np.testing.assert_array_equal(sils[:, 0:3], ref_sil[:, 3:6])   # a = m1
@@ -162,7 +171,7 @@ def test_snch_pair_layout(name):
         i = stack.pop()
         for j in range(2):
             ref = int(pf[i]["child"][j])
-            assert ref == int(pq[i]["child"][j])
+            assert pq is None or ref == int(pq[i]["child"][j])
             if ref & LEAF:
                 a, c = leaf_range(ref, big)
                 assert a + c <= len(sils)
@@ -172,6 +181,15 @@ def test_snch_pair_layout(name):
                 stack.append(ref)
             sl = slots(ref)
             lo, hi = pf[i]["lo"][j], pf[i]["hi"][j]
+            if D == 2:
+                # the record carries the reference node's own float32 box and cone
+                geo = sils[sl, 2:4]   # the silhouette vertex b of each (a, b, c) record
+                assert (geo >= lo).all() and (geo <= hi).all()
+                rec = np.concatenate([lo, hi, pf[i]["cone_axis"][j], [pf[i]["half"][j], pf[i]["radius"][j]]])
+                assert rec.astype(np.float32).tobytes() in node_keys, (i, j)
+                half = pf[i]["half"][j]
+                assert pf[i]["ref"][j] == (0 if 0.0 <= half <= np.float32(np.pi / 2) else -1)
+                continue
             qlo = decode(pq[i]["qlo"][j], pq[i]["exp"], pq[i]["origin"])
             qhi = decode(pq[i]["qhi"][j], pq[i]["exp"], pq[i]["origin"])
             assert (qlo <= lo).all() and (qhi >= hi).all()
