@@ -403,15 +403,12 @@ def main():
     if args.tune:
         import torch
         from gquery_amd import synth, tune
-        from gquery_amd.shapes.mesh import Mesh
-        from gquery_amd.shapes.polyline import Polyline
         torch.cuda.set_device(0)
         dim = 2 if args.shape == "polyline" else 3
         V, F = synth.star_polyline(args.segments) if dim == 2 else synth.displaced_icosphere(args.subdiv)
         x = torch.from_numpy(synth.uniform_points(args.n, dim)).cuda()
         v = torch.from_numpy(synth.uniform_directions(args.n, dim)).cuda()
-        res = tune.tune_shape(Polyline if dim == 2 else Mesh, V, F, queries={args.query: (x, v)},
-                              kinds=(args.query,), log=log)
+        res = tune.tune_shape(V, F, queries={args.query: (x, v)}, kinds=(args.query,), log=log)
         print(json.dumps({"shape": args.shape, "query": args.query, "n": args.n, "best": res["options"],
                           "ms": res["ms"], "default_ms": [t["ms"] for t in res["trials"]
                                                           if t["leaf_prims"] == 4 and t["packet_budget"] == 1500

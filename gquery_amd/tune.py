@@ -8,7 +8,7 @@ Knobs (all exact: every candidate returns the same records):
 The traversal stack depth is not a knob: it is fixed by the tree (the engine sizes the LDS stacks
 and frontiers from the scene's depth at launch).
 
-    best = tune_shape(Mesh, V, F, queries={"cp": x_cp, "sil": x_sil})
+    best = tune_shape(V, F, queries={"cp": x_cp, "sil": x_sil})
     mesh = Mesh(V, F, **best["options"])        # leaf_prims, packet_budget, traversal
 
 Timing uses a sample of the caller's queries (or uniform points in the scene box enlarged by
@@ -41,12 +41,12 @@ def _time(fn, reps=3):
     return float(np.median(ts))
 
 
-def _run(shape, kind, x, v):
+def _run(scene, kind, x, v):
     if kind == "cp":
-        return shape.closest_point_bvh(x)
+        return scene.closest_point(x)
     if kind == "ray":
-        return shape.intersect_bvh(x, v)
-    return shape.closest_silhouette_snch(x)
+        return scene.intersect(x, v)
+    return scene.closest_silhouette(x)
 
 
 def sample_queries(V, n, seed=0):
@@ -61,13 +61,18 @@ def sample_queries(V, n, seed=0):
     return x, d
 
 
-def tune_shape(cls, V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1 << 18,
+def tune_shape(V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1 << 18,
                device="cuda:0", leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, log=None):
-    """Returns {"options": {...}, "ms": {kind: ms}, "trials": [...]} for a Mesh / Polyline class.
+    """Returns {"options": {...}, "ms": {kind: ms}, "trials": [...]} for the polyline (V: n x 2) or
+    mesh (V: n x 3) with primitives F.
 
     queries: {kind: (x, v)} device tensors to time on (a sample of the real workload); otherwise
-    uniform points.  Cost: len(leaves) scene builds and len(leaves)*(len(budgets)+len(traversals))
-    timed calls per kind on n_sample queries."""
+    uniform points.  Cost: one CPU build of the SNCH, one device scene (engine layout + upload) per
+    candidate, and a few timed calls per candidate and kind."""
+    from . import ext
+    from .scene import Scene
+    dim = np.asarray(V).shape[1]
+    structure = (ext.SNCH3D if dim == 3 else ext.SNCH)(V, F)
     if queries is None:
         x, v = sample_queries(V, n_sample)
         x, v = torch.from_numpy(x).to(device), torch.from_numpy(v).to(device)
@@ -77,17 +82,17 @@ def tune_shape(cls, V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: in
         for budget, trav in itertools.product(budgets, traversals):
             if trav == 2 and budget != budgets[0]:
                 continue        # lane per query ignores the budget: time it once
-            shape = cls(V, F, device=device, leaf_prims=leaf, packet_budget=budget, traversal=trav)
+            scene = Scene(structure, device, leaf, budget, trav)
             ms = {}
             for k in kinds:
                 if k != "sil" and trav == 2:
                     continue
                 x, v = queries[k]
-                ms[k] = _time(lambda: _run(shape, k, x, v))
+                ms[k] = _time(lambda: _run(scene, k, x, v))
             trials.append({"leaf_prims": leaf, "packet_budget": budget, "traversal": trav, "ms": ms})
             if log:
                 log(f"tune leaf={leaf} budget={budget} traversal={trav}: {ms}")
-            del shape
+            del scene
     # leaf size and budget: minimise the summed closest-point + ray time; silhouettes pick their
     # traversal (and budget) on the chosen leaf size
     fast = [k for k in kinds if k != "sil"]

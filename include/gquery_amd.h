@@ -16,6 +16,9 @@
  * a negative status; gq_last_error() gives the message of the last failure on the calling thread.
  * Scene buffers are read-only after gq_scene_create, so concurrent queries on distinct streams
  * against one scene are safe.
+ *
+ * Environment: GQ_XCD_SPREAD=1 hands each of the 8 XCDs one contiguous run of the engine's query
+ * tiles instead of dealing tiles round robin (read once per process; default 0).
  */
 #ifndef GQUERY_AMD_H
 #define GQUERY_AMD_H

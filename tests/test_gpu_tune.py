@@ -1,0 +1,35 @@
+"""GPU: the autotuner picks among exact candidates; its choice returns the default records."""
+import numpy as np
+import pytest
+import torch
+
+from gquery_amd import synth, tune
+from gquery_amd.shapes.mesh import Mesh
+from gquery_amd.shapes.polyline import Polyline
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("valid", "p", "d", "prim_id")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tuned_options_give_the_default_records(gpu, dim):
+    V, F = synth.star_polyline(4096, seed=8) if dim == 2 else synth.displaced_icosphere(4, seed=9)
+    x = torch.from_numpy(synth.uniform_points(20000, dim, seed=31)).cuda()
+    v = torch.from_numpy(synth.uniform_directions(20000, dim, seed=32)).cuda()
+    res = tune.tune_shape(V, F, queries={k: (x, v) for k in ("cp", "ray", "sil")},
+                          leaves=(2, 4), budgets=(1, 1000), traversals=(1, 2))
+    opt = res["options"]
+    assert opt["leaf_prims"] in (2, 4) and opt["packet_budget"] in (1, 1000) and opt["traversal"] in (1, 2)
+    assert len(res["trials"]) == 2 * 3 and set(res["ms"]) == {"cp", "ray", "sil"}
+    cls = Polyline if dim == 2 else Mesh
+    base, tuned = cls(V, F, device="cuda:0"), cls(V, F, device="cuda:0", **opt)
+    for q in ("cp", "ray", "sil"):
+        rec = []
+        for shape in (base, tuned):
+            r = (shape.closest_point_bvh(x) if q == "cp" else shape.intersect_bvh(x, v) if q == "ray"
+                 else shape.closest_silhouette_snch(x))
+            torch.cuda.synchronize()
+            rec.append({f: getattr(r, f).cpu().numpy() for f in FIELDS})
+        for f in FIELDS:
+            assert np.array_equal(rec[0][f], rec[1][f]), (q, f)
