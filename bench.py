@@ -367,6 +367,8 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--leaf-prims", type=int, default=0, help="engine BVH4 leaf size (0 = default)")
     ap.add_argument("--traversal", type=int, default=0, help="silhouettes: 0 auto, 1 packet, 2 lane per query")
+    ap.add_argument("--wave-front", type=int, default=0,
+                    help="first wave-per-query pass frontier entries (0 = default)")
     ap.add_argument("--packet-budget", type=int, default=0,
                     help="packet steps per wavefront before deferral (0 = default, -1 = never)")
     ap.add_argument("--query", default="cp", choices=["cp", "ray", "sil"],
@@ -442,11 +444,11 @@ def main():
         dim = 2
         V, F = synth.star_polyline(args.segments)
         mesh = Polyline(V, F, device="cuda:0", leaf_prims=args.leaf_prims, packet_budget=args.packet_budget,
-                       traversal=args.traversal)
+                       traversal=args.traversal, wave_front=args.wave_front)
     else:
         V, F = synth.displaced_icosphere(args.subdiv)
         mesh = Mesh(V, F, device="cuda:0", leaf_prims=args.leaf_prims, packet_budget=args.packet_budget,
-                       traversal=args.traversal)
+                    traversal=args.traversal, wave_front=args.wave_front)
     log(f"scene built in {time.perf_counter() - t0:.1f}s: {F.shape[0]} primitives")
     nq = args.n
     x_host = synth.uniform_points(nq, dim)

@@ -209,6 +209,7 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     const int leaf_prims = (opt && opt->leaf_prims > 0) ? opt->leaf_prims : kDefaultLeafPrims;
     if (leaf_prims > 16) return fail(GQ_ERR_INVALID, "leaf_prims must be in 1..16");
     if (opt && (opt->traversal < 0 || opt->traversal > 2)) return fail(GQ_ERR_INVALID, "traversal must be 0, 1 or 2");
+    if (opt && (opt->wave_front < 0 || opt->wave_front > 8192)) return fail(GQ_ERR_INVALID, "wave_front must be 0..8192");
     *out = nullptr;
     const HostStructure &h = st->h;
     std::unique_ptr<gq_scene> s(new (std::nothrow) gq_scene());
@@ -247,6 +248,7 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     wv.wide_depth = wh.wide_depth;
     wv.packet_budget = opt ? opt->packet_budget : 0;
     wv.traversal = opt ? opt->traversal : 0;
+    wv.wave_front = opt ? opt->wave_front : 0;
     wv.bin_depth = h.max_depth;
     wv.n_nodes = (int64_t)wh.n_nodes;
     wv.n_snch_nodes = (int64_t)wh.n_snch_nodes;

@@ -45,6 +45,7 @@ struct WideView {
     float scene_lo[3], scene_hi[3];
     int packet_budget;        // 0 = default, < 0 = no deferral to the wave-per-query pass
     int traversal;            // silhouettes: 0 auto, 1 packet + wave pass, 2 lane per query
+    int wave_front;           // first wave-pass frontier entries (0 = default)
 };
 
 struct QueryArgs {

@@ -46,7 +46,8 @@ def _ptr(t):
 class Scene:
     """One polyline or mesh on the device: reference-layout BVH/SNCH and the engine layouts."""
 
-    def __init__(self, structure, device=None, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0):
+    def __init__(self, structure, device=None, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0,
+                 wave_front: int = 0):
         self.device = _device(device)
         self.dim = structure.info()["dim"]
         self.has_snch = bool(structure.info()["is_snch"])
@@ -54,7 +55,7 @@ class Scene:
         h = C.c_void_p()
         with torch.cuda.device(self.device):
             opt = _lib.SceneOptions(leaf_prims=int(leaf_prims), packet_budget=int(packet_budget),
-                                    traversal=int(traversal))
+                                    traversal=int(traversal), wave_front=int(wave_front))
             _lib.check(_lib.lib().gq_scene_create_ex(structure.handle, C.byref(opt), C.byref(h)), "gq_scene_create")
         self._h = h
         # held by the object: module globals may already be gone when it is collected at exit

@@ -11,12 +11,12 @@ from ..scene import INF, Scene
 
 class Mesh:
     def __init__(self, vertices, indices, device=None, max_prims_in_node: int = 10, leaf_prims: int = 0,
-                 packet_budget: int = 0, traversal: int = 0):
+                 packet_budget: int = 0, traversal: int = 0, wave_front: int = 0):
         self.snch = ext.SNCH3D(vertices, indices, max_prims_in_node)
         self.bvh = self.snch
         self.vertices = self.snch._vertices
         self.indices = self.snch._indices
-        self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal)
+        self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal, wave_front)
 
     def intersect_bvh(self, p, v, r_max=INF, method: str = "engine"):
         """BVH3D.intersect (bvh3d.py:176-239) -> Intersection3D."""

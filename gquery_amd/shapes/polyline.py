@@ -14,13 +14,13 @@ from ..scene import INF, Scene
 
 class Polyline:
     def __init__(self, vertices, indices, device=None, max_prims_in_node: int = 10, leaf_prims: int = 0,
-                 packet_budget: int = 0, traversal: int = 0):
+                 packet_budget: int = 0, traversal: int = 0, wave_front: int = 0):
         # One SNCH build serves both queries: its BVH part is the same tree BVH() builds.
         self.snch = ext.SNCH(vertices, indices, max_prims_in_node)
         self.bvh = self.snch
         self.vertices = self.snch._vertices
         self.indices = self.snch._indices
-        self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal)
+        self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal, wave_front)
 
     def intersect_bvh(self, p, v, r_max=INF, method: str = "engine"):
         """BVH.intersect (bvh.py:200-276) -> Intersection."""

@@ -100,7 +100,11 @@ typedef struct gq_scene_options {
                               3D / 1000 in 2D, silhouettes 1000), < 0 = never */
     int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D, packet + wave
                               pass in 3D), 1 = packet + wave pass, 2 = lane per query */
-    int32_t reserved[5];
+    int32_t wave_front;    /* LDS frontier entries of the first wave-per-query pass, 1..8192;
+                              0 = default (closest point 1024, silhouettes 640).  A query whose
+                              frontier outgrows it is finished by a second pass with the full
+                              frontier: results do not change */
+    int32_t reserved[4];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
 /* info[0..7] = dim, has_snch, device bytes, n_nodes, n_wide_nodes, n_prims, n_silhouettes, 0 */

@@ -221,18 +221,22 @@ def test_edge_cases(gpu, method, dim):
         assert res["valid_mismatch"] == 0 and res["max_err_d"] < 1e-5, res
 
 
+@pytest.mark.parametrize("wave_front", [0, 40])
 @pytest.mark.parametrize("budget", [1, 6])
 @pytest.mark.parametrize("name", ["star4096", "ico5"])
 @pytest.mark.parametrize("kind", ["cp", "sil"])
 @pytest.mark.parametrize("method", ["engine_fp32", "engine"])
-def test_wave_per_query_pass(gpu, budget, name, kind, method):
+def test_wave_per_query_pass(gpu, wave_front, budget, name, kind, method):
     """A tiny packet step budget sends (nearly) every query through the wave-per-query pass, which
-    starts from the packet pass's partial answer; results must meet the same rule as above."""
+    starts from the packet pass's partial answer; a 40-entry first-pass frontier hands most of them
+    on to the full-frontier pass, which restarts from the best answer so far.  Results must meet
+    the same rule as above."""
     from gquery_amd.shapes.mesh import Mesh
     from gquery_amd.shapes.polyline import Polyline
     V, F, base = scene(name)
     dim = V.shape[1]
-    shape = (Polyline if dim == 2 else Mesh)(V, F, device="cuda:0", packet_budget=budget)
+    shape = (Polyline if dim == 2 else Mesh)(V, F, device="cuda:0", packet_budget=budget, traversal=1,
+                                             wave_front=wave_front)
     x = synth.uniform_points(3000, dim, seed=31)
     for r_max in (float("inf"), 0.25):
         if kind == "cp" and r_max != float("inf"):
