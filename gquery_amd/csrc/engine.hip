@@ -774,14 +774,15 @@ struct PairChild {
     uint32_t child;
     int32_t ref_cone;
     float half, radius;
+    float sin_h, cos_h;   // 2D: sin and cos of the reference half-angle
 };
 
 template <int D, bool Q>
 __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairChild (&c)[2]) {
     if constexpr (D == 2) {
         // SPairF<2> (128 B, both node formats): lo[2][2] @0, hi[2][2] @16, cone axis @32, half @48,
-        // radius @56, child @64, ref_cone @72, cull @80
-        Words<24> r;
+        // radius @56, child @64, ref_cone @72, cull @80, sin(half) @88, cos(half) @96
+        Words<28> r;
         r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 128);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
@@ -798,6 +799,8 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
             c[j].ref_cone = (int32_t)r.u32(72 + 4 * j);
             c[j].cos_t = r.u32(80 + 4 * j) ? 2.0f : 0.0f;
             c[j].sin_t = 1.0f;
+            c[j].sin_h = r.f32(88 + 4 * j);
+            c[j].cos_h = r.f32(96 + 4 * j);
         }
     } else if constexpr (Q) {
         // SPairQ<3> (64 B): origin, exp | child boxes | cone_axis int16 @28 | cone_cs int16 @40 |
@@ -888,11 +891,45 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
 // The reference cone test (SNCHNode.overlaps, snch.py) of a child whose reference cone can cull,
 // evaluated with the reference's float32 box and cone: inline in the 2D pair record, from the
 // reference-cone table in 3D.  d_min receives the box distance.
+// 2D reference cone test with a fast path that needs no transcendental.  With c the cosine between
+// the cone axis and the view direction to the box centre, the reference's tests
+// |acos(c) - pi/2| <= half and |acos(c) - pi/2| <= half + asin(radius / l) are, in exact
+// arithmetic, |c| <= sin(half) and |c| <= sin(half) sqrt(1 - rho^2) + cos(half) rho
+// (rho = radius / l < cos(half); rho > cos(half) means the widened half-angle exceeds pi/2: an
+// overlap).  Where |c| or rho is further than 4e-6 from a threshold (ten times the float32
+// rounding of either side, acos and asin included) the outcome equals the reference's float32
+// evaluation, which runs for everything else.
+__device__ __forceinline__ bool ref_cone_2d(const PairChild &c, V2 x, float r, float &d_min) {
+    const V2 lo = v2(c.lo[0], c.lo[1]), hi = v2(c.hi[0], c.hi[1]);
+    float d_max;
+    box_distance(lo, hi, x, d_min, d_max);
+    if (!(d_min <= r)) return false;
+    constexpr float m = 4e-6f;
+    const V2 va = (lo + hi) / 2.0f - x;
+    const float l2 = dot(va, va);
+    const float il = rsqrtf(l2);
+    const float cs = fabsf(c.axis[0] * va.x + c.axis[1] * va.y) * il;
+    if (cs <= c.sin_h - m) return true;
+    if (cs >= c.sin_h + m) {
+        const float l = l2 * il;
+        if (l > c.radius * (1.0f + m)) {
+            const float rho = c.radius * il;
+            if (rho >= c.cos_h + m) return true;
+            if (rho <= c.cos_h - m) {
+                const float t2 = c.sin_h * sqrtf(fmaxf(1.0f - rho * rho, 0.0f)) + c.cos_h * rho;
+                if (cs <= t2 - m) return true;
+                if (cs >= t2 + m) return false;
+            }
+        }
+    }
+    return snch_overlaps(lo, hi, v2(c.axis[0], c.axis[1]), c.half, c.radius, x, r, d_min);
+}
+
 template <int D>
 __device__ __forceinline__ bool ref_cone_pass(const GP &P, const PairChild &c, const typename VT<D>::T &x, float r,
                                               float &d_min) {
     if constexpr (D == 2) {
-        return snch_overlaps(ldv<D>(c.lo), ldv<D>(c.hi), v2(c.axis[0], c.axis[1]), c.half, c.radius, x, r, d_min);
+        return ref_cone_2d(c, x, r, d_min);
     } else {
         const RefCone rc = ldg(P.ref_cones + c.ref_cone);
         return snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, d_min);

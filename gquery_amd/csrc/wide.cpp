@@ -349,6 +349,8 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 }
                 pf.cone_half[j] = n[6];
                 pf.cone_radius[j] = n[7];
+                pf.cone_sin[j] = (float)std::sin((double)n[6]);
+                pf.cone_cos[j] = (float)std::cos((double)n[6]);
                 pf.child[j] = child_ref(c, pidx);
                 // the reference cone test can only cull when 0 <= half <= pi/2 (a wider cone always
                 // contains a direction perpendicular to every view direction; a negative one is unset)

@@ -105,7 +105,8 @@ template <> struct alignas(64) SPairF<2> {
     uint32_t child[2];
     int32_t ref_cone[2];                 // 0 = the reference cone test can cull, -1 = it cannot
     int32_t cull[2];                     // 1 = no silhouette below this child (always skipped)
-    uint8_t pad[40];
+    float cone_sin[2], cone_cos[2];      // sin and cos of the half-angle (the kernels' fast path)
+    uint8_t pad[24];
 };
 static_assert(sizeof(SPairF<3>) == 128 && sizeof(SPairF<2>) == 128, "SPairF is two lines");
 
@@ -131,7 +132,8 @@ static_assert(offsetof(SPairQ<3>, ref_cone) == 56, "pair ref");
 static_assert(offsetof(SPairF<3>, cone) == 48 && offsetof(SPairF<3>, child) == 88 && offsetof(SPairF<3>, ref_cone) == 96, "SPairF<3>");
 static_assert(offsetof(SPairF<2>, cone_axis) == 32 && offsetof(SPairF<2>, cone_half) == 48 &&
               offsetof(SPairF<2>, cone_radius) == 56 && offsetof(SPairF<2>, child) == 64 &&
-              offsetof(SPairF<2>, ref_cone) == 72 && offsetof(SPairF<2>, cull) == 80, "SPairF<2>");
+              offsetof(SPairF<2>, ref_cone) == 72 && offsetof(SPairF<2>, cull) == 80 &&
+              offsetof(SPairF<2>, cone_sin) == 88 && offsetof(SPairF<2>, cone_cos) == 96, "SPairF<2>");
 static_assert(offsetof(QNode<3>, qlo) == 16 && offsetof(QNode<3>, qhi) == 28 && offsetof(QNode<3>, child) == 40, "QNode<3>");
 static_assert(offsetof(QNode<2>, nchild) == 10 && offsetof(QNode<2>, qlo) == 12 && offsetof(QNode<2>, qhi) == 28 &&
               offsetof(QNode<2>, child) == 44, "QNode<2>");

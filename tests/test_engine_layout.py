@@ -33,7 +33,7 @@ def spf_dtype(D):
     if D == 2:
         return np.dtype([("lo", "<f4", (2, 2)), ("hi", "<f4", (2, 2)), ("cone_axis", "<f4", (2, 2)),
                          ("half", "<f4", 2), ("radius", "<f4", 2), ("child", "<u4", 2), ("ref", "<i4", 2),
-                         ("cull", "<i4", 2), ("pad", "u1", 40)])
+                         ("cull", "<i4", 2), ("sin", "<f4", 2), ("cos", "<f4", 2), ("pad", "u1", 24)])
     return np.dtype([("lo", "<f4", (2, D)), ("hi", "<f4", (2, D)), ("axis0", "<f4", 3), ("cos0", "<f4"),
                      ("sin0", "<f4"), ("axis1", "<f4", 3), ("cos1", "<f4"), ("sin1", "<f4"),
                      ("child", "<u4", 2), ("ref", "<i4", 2), ("pad", "u1", 128 - (4 * D * 4 + 56))])
@@ -189,6 +189,8 @@ def test_snch_pair_layout(name):
                 assert rec.astype(np.float32).tobytes() in node_keys, (i, j)
                 half = pf[i]["half"][j]
                 assert pf[i]["ref"][j] == (0 if 0.0 <= half <= np.float32(np.pi / 2) else -1)
+                assert abs(pf[i]["sin"][j] - np.sin(np.float64(half))) <= 1e-7
+                assert abs(pf[i]["cos"][j] - np.cos(np.float64(half))) <= 1e-7
                 continue
             qlo = decode(pq[i]["qlo"][j], pq[i]["exp"], pq[i]["origin"])
             qhi = decode(pq[i]["qhi"][j], pq[i]["exp"], pq[i]["origin"])
