@@ -412,9 +412,7 @@ def main():
         v = torch.from_numpy(synth.uniform_directions(args.n, dim)).cuda()
         res = tune.tune_shape(V, F, queries={args.query: (x, v)}, kinds=(args.query,), log=log)
         print(json.dumps({"shape": args.shape, "query": args.query, "n": args.n, "best": res["options"],
-                          "ms": res["ms"], "default_ms": [t["ms"] for t in res["trials"]
-                                                          if t["leaf_prims"] == 4 and t["packet_budget"] == 1500
-                                                          and t["traversal"] == 1]}))
+                          "ms": res["ms"], "trials": res["trials"]}))
         return
     if args.batched:
         batched(args)

@@ -46,8 +46,11 @@ constexpr int kRayStackMax = 96;  // lane-stack entries: 96 * 64 lanes * 8 B = 4
 constexpr int kWavePassBlocks = 2048;  // persistent grid of the full-frontier wave-per-query pass
 constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its own L2
 // Packet steps per wavefront before its queries are deferred to the wave-per-query pass, by query
-// kind and dimension (measured optimum for 4-primitive leaves; a scene option overrides it).
-constexpr int kPacketBudgetCp3 = 1500, kPacketBudgetCp2 = 1000, kPacketBudgetSil = 1000;
+// kind and dimension (measured optimum with 6-primitive leaves and the two-phase wave pass on 1M
+// uniform queries: 3D closest point 7.73 ms at 1000 vs 8.32 at 800 and 7.92 at 1200; 2D closest
+// point 2.44 ms at 600 vs 2.52 at 750; silhouettes flat within 1% over 400-1000).  A scene or job
+// option overrides it.
+constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil = 750;
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
 
