@@ -50,3 +50,36 @@ def summarize(name, got, ref, gap=None, tol=TOL_REF, vec_fields=("p",), scalar_f
     out["ok"] = (out["valid_mismatch"] == 0 and out["id_mismatch"] == 0
                  and all(out[k] <= tol for k in out if k.startswith("max_err_")))
     return out
+
+
+def sil2d_fp64_predicate(silhouette_data, ids, x):
+    """FP64 silhouette-vertex predicate cross(b-a, x-a) * cross(c-b, x-b) < 0 of the vertices `ids`
+    (the 2D records' prim_id) for query points x, from the flat 2D silhouette records (stride 10:
+    a, b, c, neighbour indices, vertex id)."""
+    sil = np.asarray(silhouette_data, np.float64).reshape(-1, 10)
+    row = {int(v): k for k, v in enumerate(sil[:, 9])}
+    r = np.array([row[int(i)] for i in ids], np.int64)
+    a, b, c = sil[r, 0:2], sil[r, 2:4], sil[r, 4:6]
+    X = np.asarray(x, np.float64).reshape(-1, 2)
+    c0 = (b[:, 0] - a[:, 0]) * (X[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (X[:, 0] - a[:, 0])
+    c1 = (c[:, 0] - b[:, 0]) * (X[:, 1] - b[:, 1]) - (c[:, 1] - b[:, 1]) * (X[:, 0] - b[:, 0])
+    return c0 * c1 < 0.0
+
+
+def sil2d_differences_explained(silhouette_data, x, eng, ref):
+    """Queries where the engine's 2D silhouette differs from the reference float32 one, and whether
+    every difference is the FP64 predicate overruling a float32 sign: the engine's nearer vertex
+    passes in FP64, or the reference's nearer vertex fails in FP64."""
+    ev, rv = np.asarray(eng["valid"]).astype(bool), np.asarray(ref["valid"]).astype(bool)
+    ed, rd = np.asarray(eng["d"]), np.asarray(ref["d"])
+    diff = np.where((ev != rv) | (ev & rv & (ed != rd)))[0]
+    eng_nearer = ev[diff] & (~rv[diff] | (ed[diff] < rd[diff]))
+    ok = np.ones(len(diff), bool)
+    x = np.asarray(x).reshape(-1, 2)
+    i = diff[eng_nearer]
+    if len(i):
+        ok[eng_nearer] = sil2d_fp64_predicate(silhouette_data, np.asarray(eng["prim_id"])[i], x[i])
+    i = diff[~eng_nearer]
+    if len(i):
+        ok[~eng_nearer] = ~sil2d_fp64_predicate(silhouette_data, np.asarray(ref["prim_id"])[i], x[i])
+    return diff, ok

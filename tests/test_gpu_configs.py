@@ -1,0 +1,159 @@
+"""GPU: the benchmark configurations at their full sizes.
+
+Config 1 (star polyline, 1,024 segments, 65,536 closest-point queries and rays) is checked query by
+query against the oracle.  Configs 2 and 3 (1M queries) are checked against the oracle and the FP64
+brute force on a sample, and on every query through properties that do not depend on the size:
+agreement between the engine and the plain kernels (the reference's float32 traversal), the record
+invariants (d = |p - x|, p on the reported primitive), and upper bounds from sampled primitives.
+"""
+import numpy as np
+import pytest
+import torch
+
+from gquery_amd import synth
+from gquery_amd.shapes.mesh import Mesh
+from gquery_amd.shapes.polyline import Polyline
+from oracle import oracle as O
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+N_FULL = 1 << 20
+
+
+def _np(r, fields=("valid", "prim_id", "d", "p", "n", "uv", "on_boundary")):
+    torch.cuda.synchronize()
+    return {f: getattr(r, f).cpu().numpy() for f in fields if hasattr(r, f)}
+
+
+def _point_triangle_distance(p, a, b, c):
+    """FP64 distance from points p to triangles abc (rows), via the projection onto the plane and
+    the three edges (a direct formula independent of the kernels' region test)."""
+    p, a, b, c = (np.asarray(t, np.float64) for t in (p, a, b, c))
+
+    def seg(p, u, v):
+        d = v - u
+        t = np.clip(((p - u) * d).sum(1) / np.maximum((d * d).sum(1), 1e-300), 0.0, 1.0)
+        return np.linalg.norm(p - (u + t[:, None] * d), axis=1)
+
+    n = np.cross(b - a, c - a)
+    nn = np.linalg.norm(n, axis=1)
+    n = n / np.maximum(nn, 1e-300)[:, None]
+    h = ((p - a) * n).sum(1)
+    q = p - h[:, None] * n
+    # q inside the triangle: all three edge cross products point along n
+    s0 = (np.cross(b - a, q - a) * n).sum(1)
+    s1 = (np.cross(c - b, q - b) * n).sum(1)
+    s2 = (np.cross(a - c, q - c) * n).sum(1)
+    inside = (s0 >= 0) & (s1 >= 0) & (s2 >= 0)
+    edge = np.minimum(np.minimum(seg(p, a, b), seg(p, b, c)), seg(p, c, a))
+    return np.where(inside, np.abs(h), edge)
+
+
+@pytest.mark.parametrize("method", ["plain", "engine_fp32", "engine"])
+def test_config1_star1024_full(gpu, method):
+    V, F = synth.star_polyline(1024)
+    shape = Polyline(V, F, device="cuda:0")
+    s = shape.snch
+    n = 65536
+    x = synth.uniform_points(n, 2)
+    v = synth.uniform_directions(n, 2)
+    for kind in ("cp", "ray"):
+        if kind == "cp":
+            got = _np(shape.closest_point_bvh(x, method=method))
+            ref = O.closest_point(2, s.bvh_node_data(), s.primitive_data(), x)
+            b = O.bf_closest_point(2, V, F, x)
+            gap = parity.gap_mask(b["d"], b["second_d"])
+        else:
+            got = _np(shape.intersect_bvh(x, v, method=method))
+            ref = O.intersect(2, s.bvh_node_data(), s.primitive_data(), x, v)
+            b = O.bf_intersect(2, V, F, x, v)
+            gap = parity.gap_mask(b["t"], b["second_t"])
+        res = parity.summarize(f"config1/{kind}/{method}", got, ref, gap, vec_fields=("p", "n"))
+        assert res["ok"], res
+        assert res["id_checked"] > n // 4
+
+
+def test_config2_icosphere_s6_full(gpu):
+    V, F = synth.displaced_icosphere(6)
+    assert F.shape[0] == 81920
+    shape = Mesh(V, F, device="cuda:0")
+    s = shape.snch
+    x = synth.uniform_points(N_FULL, 3)
+    v = synth.uniform_directions(N_FULL, 3)
+    xt, vt = torch.from_numpy(x).cuda(), torch.from_numpy(v).cuda()
+    rng = np.random.default_rng(3)
+    sample = rng.choice(N_FULL, 16384, replace=False)
+
+    # closest point: engine vs plain on every query, records self-consistent, oracle on a sample
+    eng = _np(shape.closest_point_bvh(xt, method="engine"))
+    pl = _np(shape.closest_point_bvh(xt, method="plain"))
+    assert eng["valid"].all() and pl["valid"].all()          # a closed mesh: every query has one
+    assert parity.rel_err(eng["d"], pl["d"]).max() <= parity.TOL_REF
+    tie = eng["prim_id"] != pl["prim_id"]                    # ties: nearest points on shared edges
+    assert (parity.rel_err(eng["d"][tie], pl["d"][tie]) <= 1e-6).all()
+    assert parity.rel_err(np.linalg.norm(eng["p"] - x, axis=1), eng["d"]).max() <= 1e-5
+    tri = F[eng["prim_id"]]
+    dp = _point_triangle_distance(eng["p"], V[tri[:, 0]], V[tri[:, 1]], V[tri[:, 2]])
+    assert dp.max() <= 1e-5                                  # p lies on its triangle
+    probe = rng.integers(0, F.shape[0], size=N_FULL)         # any triangle bounds d from above
+    dprobe = _point_triangle_distance(x, V[F[probe, 0]], V[F[probe, 1]], V[F[probe, 2]])
+    assert (eng["d"] <= dprobe * (1 + 1e-5) + 1e-6).all()
+    ref = O.closest_point(3, s.bvh_node_data(), s.primitive_data(), x[sample])
+    b = O.bf_closest_point(3, V, F, x[sample])
+    got = {k: a[sample] for k, a in eng.items()}
+    res = parity.summarize("config2/cp", got, ref, parity.gap_mask(b["d"], b["second_d"]), vec_fields=("p", "n"))
+    assert res["ok"], res
+    assert parity.rel_err(got["d"], b["d"]).max() <= parity.TOL_F64
+
+    # rays: engine vs plain on every ray (the engine's FP64 band can only make it closer to FP64)
+    eng = _np(shape.intersect_bvh(xt, vt, method="engine"))
+    pl = _np(shape.intersect_bvh(xt, vt, method="plain"))
+    ev, pv = eng["valid"].astype(bool), pl["valid"].astype(bool)
+    assert (ev != pv).mean() < 1e-4
+    both = ev & pv
+    assert np.median(parity.rel_err(eng["d"][both], pl["d"][both])) == 0.0
+    assert parity.rel_err(eng["d"][both], pl["d"][both]).max() <= 1e-3
+    hit = eng["p"][ev]
+    assert parity.rel_err(np.linalg.norm(hit - x[ev], axis=1), eng["d"][ev]).max() <= 1e-4
+    b = O.bf_intersect(3, V, F, x[sample], v[sample])
+    bv = np.isfinite(b["t"])
+    assert np.array_equal(ev[sample], bv)
+    assert parity.rel_err(eng["d"][sample][bv], b["t"][bv]).max() <= parity.TOL_F64
+
+
+@pytest.mark.parametrize("method", ["engine_fp32", "engine"])
+def test_config3_silhouettes_full(gpu, method):
+    # 3D: icosphere s=7, 1M queries; the plain kernel (3.9 s per 65,536) and the oracle on a sample
+    V, F = synth.displaced_icosphere(7)
+    shape = Mesh(V, F, device="cuda:0")
+    x = synth.uniform_points(N_FULL, 3)
+    got = _np(shape.closest_silhouette_snch(torch.from_numpy(x).cuda(), method=method), ("valid", "prim_id", "d", "p"))
+    assert got["valid"].mean() > 0.7
+    gv = got["valid"].astype(bool)
+    assert parity.rel_err(np.linalg.norm(got["p"][gv] - x[gv], axis=1), got["d"][gv]).max() <= 1e-5
+    rng = np.random.default_rng(4)
+    sample = rng.choice(N_FULL, 4096, replace=False)
+    xs = x[sample]
+    pl = _np(shape.closest_silhouette_snch(torch.from_numpy(xs).cuda(), method="plain"), ("valid", "prim_id", "d", "p"))
+    sub = {k: a[sample] for k, a in got.items()}
+    b = O.bf_silhouette(3, shape.snch.raw_silhouette_data(), xs)
+    bv = np.isfinite(b["d"])
+    sv, pv = sub["valid"].astype(bool), pl["valid"].astype(bool)
+    # vs the plain kernel and the FP64 brute force; the engine's FP64 edge predicate may accept a
+    # nearer edge whose float32 sign is wrong (at most 2 of 4,096 queries, always the FP64 answer)
+    off = (sv != pv) | (sv & pv & (parity.rel_err(sub["d"], pl["d"]) > parity.TOL_REF))
+    assert off.sum() <= 2 and (sv[off] & (~pv[off] | (sub["d"][off] < pl["d"][off]))).all()
+    assert np.array_equal(sv, bv)
+    assert parity.rel_err(sub["d"][bv], b["d"][bv]).max() <= parity.TOL_F64
+
+    # 2D: star with 65,536 segments, 1M queries against the plain kernel on every query
+    V2, F2 = synth.star_polyline(65536)
+    poly = Polyline(V2, F2, device="cuda:0")
+    x2 = torch.from_numpy(synth.uniform_points(N_FULL, 2)).cuda()
+    e = _np(poly.closest_silhouette_snch(x2, method=method), ("valid", "d", "prim_id"))
+    p = _np(poly.closest_silhouette_snch(x2, method="plain"), ("valid", "d", "prim_id"))
+    # differences only where the FP64 vertex predicate overrules a float32 sign (nearly collinear
+    # neighbours), and each one is the FP64 answer
+    diff, ok = parity.sil2d_differences_explained(poly.snch.silhouette_data(), x2.cpu().numpy(), e, p)
+    assert len(diff) < 1e-4 * N_FULL and ok.all(), (len(diff), diff[~ok][:5])
