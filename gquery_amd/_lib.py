@@ -90,6 +90,14 @@ SIGNATURES = {
     "gq_structure_engine_layout": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, _i64p]),
     "gq_batch_sizes": (C.c_int, [C.c_int, C.POINTER(Job), _i64p, _i64p]),
     "gq_batch_enqueue": (C.c_int, [C.c_int, C.POINTER(Job), C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]),
+    "gq_shape_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
+    "gq_shape_destroy": (C.c_int, [C.c_void_p]),
+    "gq_shape_info": (C.c_int, [C.c_void_p, _i64p]),
+    "gq_shape_normals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gq_closest_point_baseline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Records), C.c_void_p]),
+    "gq_intersect_baseline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
+    "gq_closest_silhouette_baseline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
+    "gq_configure_silhouette": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, _i64p]),
     "gq_profile_begin": (C.c_int, []),
     "gq_profile_end": (C.c_int, [C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
 }

@@ -27,6 +27,7 @@ HIP_FLAGS = [
 # CPU restatement of the reference traversal bit-for-bit, so they must not contract a*b+c.
 HIP_UNITS = {
     "plain.hip": ["-ffp-contract=off"],
+    "baseline.hip": ["-ffp-contract=off"],
     "engine.hip": ["-ffp-contract=off"],
     "gq_api.hip": [],
 }

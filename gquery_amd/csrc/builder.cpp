@@ -573,6 +573,59 @@ bool build_structure(int dim, const float *vertices, int64_t n_vertices, const i
     return build_impl<3>(vertices, n_vertices, indices, n_prims, max_prims_in_node, with_snch, out, err);
 }
 
+bool validate_shape(int dim, const float *V, int64_t nv, const int32_t *I, int64_t np, std::string &err) {
+    return validate(dim, V, nv, I, np, err);
+}
+
+bool configure_silhouette_2d(const float *V, int64_t nv, const int32_t *E, int64_t ne, std::vector<float> &out,
+                             std::string &err) {
+    if (!validate(2, V, nv, E, ne, err)) return false;
+    // neighbours of each vertex: the last edge leaving it sets `next`, the last entering it `prev`
+    std::vector<int32_t> prev((size_t)nv, -2), next((size_t)nv, -2), order;
+    order.reserve((size_t)nv);
+    for (int64_t k = 0; k < ne; ++k) {
+        const int32_t a = E[2 * k], b = E[2 * k + 1];
+        for (int32_t v : {a, b}) {
+            if (prev[(size_t)v] == -2) {
+                prev[(size_t)v] = next[(size_t)v] = -1;
+                order.push_back(v);
+            }
+        }
+        next[(size_t)a] = b;
+        prev[(size_t)b] = a;
+    }
+    out.clear();
+    // float32 unit direction, as the query kernels normalise: v * (1 / sqrt(dot(v, v)))
+    auto unit = [](float x, float y, float &ux, float &uy) {
+        const float inv = 1.0f / std::sqrt(std::fma(y, y, x * x));
+        ux = x * inv;
+        uy = y * inv;
+    };
+    for (size_t i = 0; i < order.size(); ++i) {
+        const int32_t v = order[i], p = prev[(size_t)v], n = next[(size_t)v];
+        if (p == -1 || n == -1) continue;
+        const float *a = V + 2 * (size_t)p, *b = V + 2 * (size_t)v, *c = V + 2 * (size_t)n;
+        float e1x, e1y, e2x, e2y;
+        unit(b[0] - a[0], b[1] - a[1], e1x, e1y);
+        unit(c[0] - b[0], c[1] - b[1], e2x, e2y);
+        const float col = std::fabs(std::fma(e1y, e2y, e1x * e2x));
+        if (!(col < 1.0f - 1e-5f)) continue;   // nearly collinear (or degenerate: NaN)
+        const float r[10] = {a[0], a[1], b[0], b[1], c[0], c[1], (float)p, (float)v, (float)n, (float)i};
+        out.insert(out.end(), r, r + 10);
+    }
+    return true;
+}
+
+void vertex_faces(const int32_t *F, int64_t nf, int64_t nv, std::vector<int32_t> &start, std::vector<int32_t> &list) {
+    start.assign((size_t)nv + 1, 0);
+    for (int64_t k = 0; k < 3 * nf; ++k) ++start[(size_t)F[k] + 1];
+    for (int64_t v = 0; v < nv; ++v) start[(size_t)v + 1] += start[(size_t)v];
+    list.assign((size_t)(3 * nf), 0);
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    for (int slot = 0; slot < 3; ++slot)
+        for (int64_t f = 0; f < nf; ++f) list[(size_t)fill[(size_t)F[3 * f + slot]]++] = (int32_t)f;
+}
+
 bool build_flat_silhouette_edges(const float *V, int64_t nv, const int32_t *F, int64_t nf,
                                  std::vector<float> &out, std::string &err) {
     if (!validate(3, V, nv, F, nf, err)) return false;

@@ -47,4 +47,20 @@ bool build_structure(int dim, const float *vertices, int64_t n_vertices, const i
 bool build_flat_silhouette_edges(const float *vertices, int64_t n_vertices, const int32_t *faces,
                                  int64_t n_faces, std::vector<float> &out, std::string &err);
 
+// Polyline.configure_silhouette (gquery/shapes/polyline.py:44-122): silhouette vertices of a
+// polyline as records of 10 floats (a, b, c, prev, vertex, next, index), a vertex with both
+// neighbours in first-appearance order, nearly collinear ones (|dot| of the unit edge directions
+// >= 1 - 1e-5) dropped.  `index` is the vertex's position in first-appearance order.
+bool configure_silhouette_2d(const float *vertices, int64_t n_vertices, const int32_t *edges, int64_t n_edges,
+                             std::vector<float> &out, std::string &err);
+
+// Faces incident to each vertex in the order Mesh.configure_normal (gquery/shapes/mesh.py:35-50)
+// accumulates them: the faces naming the vertex first, then second, then third, each in face
+// order.  CSR: faces of vertex v are faces[start[v] .. start[v + 1]).
+void vertex_faces(const int32_t *faces, int64_t n_faces, int64_t n_vertices, std::vector<int32_t> &start,
+                  std::vector<int32_t> &list);
+
+bool validate_shape(int dim, const float *vertices, int64_t n_vertices, const int32_t *indices, int64_t n_prims,
+                    std::string &err);
+
 } // namespace gqa

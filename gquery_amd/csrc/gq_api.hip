@@ -14,6 +14,15 @@ struct gq_structure {
     gqa::HostStructure h;
 };
 
+struct gq_shape {
+    int dim = 0;
+    int device = 0;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+    gqa::ShapeView view{};
+    float *face_normals = nullptr, *vertex_normals = nullptr;
+};
+
 struct gq_scene {
     int dim = 0;
     bool has_snch = false;
@@ -76,8 +85,8 @@ static int hip_fail(hipError_t e, const char *what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
-template <typename T>
-static int upload(gq_scene *s, const T *host, size_t count, const T **dev) {
+template <typename O, typename T>
+static int upload(O *s, const T *host, size_t count, const T **dev) {
     *dev = nullptr;
     if (count == 0) return GQ_OK;
     void *p = nullptr;
@@ -458,6 +467,146 @@ int gq_profile_end(double *total_ms, int64_t *launches, char *name, int32_t name
     }
     p.ev.clear();
     return GQ_OK;
+}
+
+// ---- Brute-force queries and normals (Polyline / Mesh *_baseline, configure_normal,
+// configure_silhouette) ----
+
+int gq_configure_silhouette(const float *V, int64_t nv, const int32_t *E, int64_t ne, float *out, int64_t *n_floats) {
+    if (!n_floats) return fail(GQ_ERR_INVALID, "n_floats is null");
+    std::vector<float> rec;
+    std::string err;
+    try {
+        if (!configure_silhouette_2d(V, nv, E, ne, rec, err)) return fail(GQ_ERR_INVALID, err);
+    } catch (const std::bad_alloc &) {
+        return fail(GQ_ERR_OOM, "host allocation failed");
+    }
+    if (out) {
+        if (*n_floats < (int64_t)rec.size()) return fail(GQ_ERR_INVALID, "output buffer too small");
+        std::memcpy(out, rec.data(), rec.size() * sizeof(float));
+    }
+    *n_floats = (int64_t)rec.size();
+    return GQ_OK;
+}
+
+int gq_shape_destroy(gq_shape *shape) {
+    if (!shape) return GQ_OK;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipSetDevice(shape->device);
+    for (void *p : shape->allocs) hipFree(p);
+    hipSetDevice(dev);
+    delete shape;
+    return GQ_OK;
+}
+
+int gq_shape_create(int dim, const float *V, int64_t nv, const int32_t *I, int64_t np, gq_shape **out) {
+    if (!out) return fail(GQ_ERR_INVALID, "out is null");
+    *out = nullptr;
+    std::string err;
+    if (!validate_shape(dim, V, nv, I, np, err)) return fail(GQ_ERR_INVALID, err);
+    std::unique_ptr<gq_shape, int (*)(gq_shape *)> s(new (std::nothrow) gq_shape(), gq_shape_destroy);
+    if (!s) return fail(GQ_ERR_OOM, "host allocation failed");
+    hipError_t e = hipGetDevice(&s->device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::vector<float> sils;
+    std::vector<int32_t> vf_start, vf_list;
+    try {
+        if (dim == 2) {
+            if (!configure_silhouette_2d(V, nv, I, np, sils, err)) return fail(GQ_ERR_INVALID, err);
+        } else {
+            if (!build_flat_silhouette_edges(V, nv, I, np, sils, err)) return fail(GQ_ERR_INVALID, err);
+            vertex_faces(I, np, nv, vf_start, vf_list);
+        }
+    } catch (const std::bad_alloc &) {
+        return fail(GQ_ERR_OOM, "host allocation failed");
+    }
+    ShapeView &v = s->view;
+    v.dim = dim;
+    v.n_vertices = nv;
+    v.n_prims = np;
+    v.n_sils = (int64_t)sils.size() / (dim == 2 ? 10 : 19);
+    int rc;
+    if ((rc = upload(s.get(), V, (size_t)(nv * dim), &v.V))) return rc;
+    if ((rc = upload(s.get(), I, (size_t)(np * dim), &v.F))) return rc;
+    if ((rc = upload(s.get(), sils.data(), sils.size(), &v.sils))) return rc;
+    if (dim == 3) {
+        if ((rc = upload(s.get(), vf_start.data(), vf_start.size(), &v.vf_start))) return rc;
+        if ((rc = upload(s.get(), vf_list.data(), vf_list.size(), &v.vf_list))) return rc;
+        void *fn = nullptr, *vn = nullptr;
+        if ((e = hipMalloc(&fn, (size_t)np * 3 * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc");
+        s->allocs.push_back(fn);
+        if ((e = hipMalloc(&vn, (size_t)nv * 3 * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc");
+        s->allocs.push_back(vn);
+        s->bytes += (size_t)(np + nv) * 3 * sizeof(float);
+        s->face_normals = (float *)fn;
+        s->vertex_normals = (float *)vn;
+        v.face_normals = s->face_normals;
+        if ((e = launch_normals(v, s->face_normals, s->vertex_normals, nullptr)) != hipSuccess)
+            return hip_fail(e, "normals");
+        if ((e = hipStreamSynchronize(nullptr)) != hipSuccess) return hip_fail(e, "normals");
+    }
+    *out = s.release();
+    return GQ_OK;
+}
+
+int gq_shape_info(const gq_shape *shape, int64_t info[4]) {
+    if (!shape || !info) return fail(GQ_ERR_INVALID, "null argument");
+    info[0] = shape->view.dim;
+    info[1] = shape->view.n_vertices;
+    info[2] = shape->view.n_prims;
+    info[3] = shape->view.n_sils;
+    return GQ_OK;
+}
+
+int gq_shape_normals(const gq_shape *shape, float *face_normals, float *vertex_normals, void *stream) {
+    if (!shape) return fail(GQ_ERR_INVALID, "null shape");
+    if (shape->view.dim != 3) return fail(GQ_ERR_UNSUPPORTED, "normals are computed for meshes (3D)");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    if (face_normals &&
+        (e = hipMemcpyAsync(face_normals, shape->face_normals, (size_t)shape->view.n_prims * 3 * sizeof(float),
+                            hipMemcpyDeviceToDevice, st)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync");
+    if (vertex_normals &&
+        (e = hipMemcpyAsync(vertex_normals, shape->vertex_normals, (size_t)shape->view.n_vertices * 3 * sizeof(float),
+                            hipMemcpyDeviceToDevice, st)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync");
+    return GQ_OK;
+}
+
+static int run_baseline(gq_shape *s, QueryKind kind, const QueryArgs &q, const float *normal,
+                        const uint8_t *on_boundary, void *stream) {
+    if (!s) return fail(GQ_ERR_INVALID, "null shape");
+    if (q.n < 0) return fail(GQ_ERR_INVALID, "negative query count");
+    if (q.n == 0) return GQ_OK;
+    if (q.n > (int64_t)INT32_MAX) return fail(GQ_ERR_INVALID, "more than 2^31-1 queries in one call");
+    if (!q.x || (kind == kIntersect && !q.v)) return fail(GQ_ERR_INVALID, "null query pointer");
+    if (on_boundary && !normal) return fail(GQ_ERR_INVALID, "on_boundary needs normals");
+    const hipError_t e = launch_baseline(kind, s->view, q, normal, on_boundary, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return GQ_OK;
+}
+
+int gq_closest_point_baseline(gq_shape *shape, const float *x, int64_t n, const gq_records *out, void *stream) {
+    if (!out) return fail(GQ_ERR_INVALID, "null records");
+    QueryArgs q{x, nullptr, nullptr, __builtin_inff(), n, *out};
+    return run_baseline(shape, kClosestPoint, q, nullptr, nullptr, stream);
+}
+
+int gq_intersect_baseline(gq_shape *shape, const float *x, const float *v, const float *normal,
+                          const uint8_t *on_boundary, const float *r_max, float r_max_scalar, int64_t n,
+                          const gq_records *out, void *stream) {
+    if (!out) return fail(GQ_ERR_INVALID, "null records");
+    QueryArgs q{x, v, r_max, r_max_scalar, n, *out};
+    return run_baseline(shape, kIntersect, q, normal, on_boundary, stream);
+}
+
+int gq_closest_silhouette_baseline(gq_shape *shape, const float *x, const float *r_max, float r_max_scalar,
+                                   int64_t n, const gq_records *out, void *stream) {
+    if (!out) return fail(GQ_ERR_INVALID, "null records");
+    QueryArgs q{x, nullptr, r_max, r_max_scalar, n, *out};
+    return run_baseline(shape, kSilhouette, q, nullptr, nullptr, stream);
 }
 
 } // extern "C"

@@ -63,6 +63,22 @@ enum QueryKind { kClosestPoint = 0, kIntersect = 1, kSilhouette = 2 };
 
 hipError_t launch_plain(QueryKind kind, const PlainView &s, const QueryArgs &q, hipStream_t st);
 
+// Device view of a polyline / mesh for the brute-force (baseline) queries and the normals.
+struct ShapeView {
+    int dim;
+    int64_t n_vertices, n_prims, n_sils;
+    const float *V;              // n_vertices x dim
+    const int32_t *F;            // n_prims x dim
+    const float *face_normals;   // 3D: n_prims x 3
+    const float *sils;           // 2D: configure_silhouette records (10 floats); 3D: flat edges (19)
+    const int32_t *vf_start;     // 3D: faces of each vertex (CSR, configure_normal order)
+    const int32_t *vf_list;
+};
+
+hipError_t launch_baseline(QueryKind kind, const ShapeView &s, const QueryArgs &q, const float *normal,
+                           const uint8_t *on_boundary, hipStream_t st);
+hipError_t launch_normals(const ShapeView &s, float *face_normals, float *vertex_normals, hipStream_t st);
+
 void set_error(const std::string &msg);
 
 // Traversal-kernel timing (gq_profile_begin / gq_profile_end); no-ops unless enabled.

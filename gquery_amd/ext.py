@@ -132,3 +132,17 @@ def build_flat_silhouette_edges(vertices, faces) -> np.ndarray:
     _lib.check(L.gq_build_flat_silhouette_edges(v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0],
                                                 out.ctypes.data, C.byref(n)))
     return out
+
+
+def configure_silhouette(vertices, edges) -> np.ndarray:
+    """Polyline.configure_silhouette (gquery/shapes/polyline.py:44-122) on the host: the polyline's
+    silhouette vertices as (k, 10) float32 records (a, b, c, prev, vertex, next, index)."""
+    v = _as_rows(vertices, 2, np.float32)
+    e = _as_rows(edges, 2, np.int32)
+    n = C.c_int64(0)
+    L = _lib.lib()
+    _lib.check(L.gq_configure_silhouette(v.ctypes.data, v.shape[0], e.ctypes.data, e.shape[0], None, C.byref(n)))
+    out = np.empty(n.value, dtype=np.float32)
+    _lib.check(L.gq_configure_silhouette(v.ctypes.data, v.shape[0], e.ctypes.data, e.shape[0], out.ctypes.data,
+                                         C.byref(n)))
+    return out.reshape(-1, 10)

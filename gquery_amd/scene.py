@@ -43,42 +43,11 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
-class Scene:
-    """One polyline or mesh on the device: reference-layout BVH/SNCH and the engine layouts."""
+class _DeviceQueries:
+    """Record allocation, r_max and stream handling shared by the device objects."""
+    dim: int
+    device: torch.device
 
-    def __init__(self, structure, device=None, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0,
-                 wave_front: int = 0):
-        self.device = _device(device)
-        self.dim = structure.info()["dim"]
-        self.has_snch = bool(structure.info()["is_snch"])
-        self._structure = structure
-        h = C.c_void_p()
-        with torch.cuda.device(self.device):
-            opt = _lib.SceneOptions(leaf_prims=int(leaf_prims), packet_budget=int(packet_budget),
-                                    traversal=int(traversal), wave_front=int(wave_front))
-            _lib.check(_lib.lib().gq_scene_create_ex(structure.handle, C.byref(opt), C.byref(h)), "gq_scene_create")
-        self._h = h
-        # held by the object: module globals may already be gone when it is collected at exit
-        self._destroy = _lib.lib().gq_scene_destroy
-        self._sync = torch.cuda.synchronize
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            try:
-                self._sync(self.device)
-            except Exception:
-                pass
-            self._destroy(h)
-            self._h = None
-
-    def info(self) -> dict:
-        a = (C.c_int64 * 8)()
-        _lib.check(_lib.lib().gq_scene_info(self._h, a))
-        keys = ["dim", "has_snch", "device_bytes", "n_nodes", "n_wide_nodes", "n_prims", "n_silhouettes"]
-        return dict(zip(keys, list(a)[:7]))
-
-    # -- helpers -------------------------------------------------------------------------------
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
@@ -113,6 +82,136 @@ class Scene:
             raise ValueError("r_max must be a scalar or have one entry per query")
         return t, INF
 
+    def _cp_record(self, out):
+        if self.dim == 3:
+            return rec.ClosestPointRecord3D(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"])
+        return rec.ClosestPointRecord(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"])
+
+    def _ray_record(self, out):
+        cls = rec.Intersection3D if self.dim == 3 else rec.Intersection
+        return cls(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"], out["on_boundary"])
+
+    def _sil_record(self, out):
+        cls = rec.ClosestSilhouettePointRecord3D if self.dim == 3 else rec.ClosestSilhouettePointRecord
+        return cls(out["valid"], out["p"], out["d"], out["prim_id"])
+
+
+class BruteForce(_DeviceQueries):
+    """A polyline or mesh on the device for the reference's brute-force methods (gq_shape): every
+    query tests every primitive / silhouette (Polyline/Mesh.*_baseline), plus the mesh normals of
+    Mesh.configure_normal.  Silhouettes: Polyline.configure_silhouette's vertices (2D) or the
+    mesh's flat silhouette edges (3D)."""
+
+    def __init__(self, dim: int, vertices, indices, device=None):
+        self.device = _device(device)
+        self.dim = int(dim)
+        V = np.ascontiguousarray(vertices, np.float32).reshape(-1, self.dim)
+        F = np.ascontiguousarray(indices, np.int32).reshape(-1, self.dim)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().gq_shape_create(self.dim, V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0],
+                                                  C.byref(h)), "gq_shape_create")
+        self._h = h
+        self._destroy = _lib.lib().gq_shape_destroy
+        self._sync = torch.cuda.synchronize
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._sync(self.device)
+            except Exception:
+                pass
+            self._destroy(h)
+            self._h = None
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 4)()
+        _lib.check(_lib.lib().gq_shape_info(self._h, a))
+        return dict(zip(["dim", "n_vertices", "n_prims", "n_silhouettes"], list(a)))
+
+    def normals(self):
+        """(face normals (F, 3), vertex normals (V, 3)) as device tensors (Mesh.configure_normal)."""
+        i = self.info()
+        with torch.cuda.device(self.device):
+            fn = torch.empty((i["n_prims"], 3), device=self.device, dtype=torch.float32)
+            vn = torch.empty((i["n_vertices"], 3), device=self.device, dtype=torch.float32)
+            _lib.check(_lib.lib().gq_shape_normals(self._h, _ptr(fn), _ptr(vn), self._stream()), "normals")
+        return fn, vn
+
+    def closest_point(self, x):
+        with torch.cuda.device(self.device):
+            x = as_points(x, self.dim, self.device)
+            n = x.shape[0]
+            out, r = self._alloc(n, ["valid", "p", "n", "uv", "d", "prim_id"])
+            _lib.check(_lib.lib().gq_closest_point_baseline(self._h, _ptr(x), n, C.byref(r), self._stream()),
+                       "closest_point_baseline")
+        return self._cp_record(out)
+
+    def intersect(self, x, v, r_max=INF, normal=None, on_boundary=None):
+        with torch.cuda.device(self.device):
+            x = as_points(x, self.dim, self.device)
+            v = as_points(v, self.dim, self.device)
+            if x.shape != v.shape:
+                raise ValueError("origins and directions must have the same shape")
+            n = x.shape[0]
+            nb = ob = None
+            if on_boundary is not None:
+                ob = torch.as_tensor(on_boundary).to(self.device, torch.bool).reshape(-1).expand(n).contiguous()
+                nb = as_points(normal if normal is not None else np.zeros(self.dim, np.float32), self.dim, self.device)
+                nb = nb.expand(n, self.dim).contiguous()
+            rt, rs = self._r_max(r_max, n)
+            out, r = self._alloc(n, ["valid", "p", "n", "uv", "d", "prim_id", "on_boundary"])
+            _lib.check(_lib.lib().gq_intersect_baseline(self._h, _ptr(x), _ptr(v), _ptr(nb), _ptr(ob), _ptr(rt), rs,
+                                                        n, C.byref(r), self._stream()), "intersect_baseline")
+        return self._ray_record(out)
+
+    def closest_silhouette(self, x, r_max=INF):
+        with torch.cuda.device(self.device):
+            x = as_points(x, self.dim, self.device)
+            n = x.shape[0]
+            rt, rs = self._r_max(r_max, n)
+            out, r = self._alloc(n, ["valid", "p", "d", "prim_id"])
+            _lib.check(_lib.lib().gq_closest_silhouette_baseline(self._h, _ptr(x), _ptr(rt), rs, n, C.byref(r),
+                                                                 self._stream()), "closest_silhouette_baseline")
+        return self._sil_record(out)
+
+
+class Scene(_DeviceQueries):
+    """One polyline or mesh on the device: reference-layout BVH/SNCH and the engine layouts."""
+
+    def __init__(self, structure, device=None, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0,
+                 wave_front: int = 0):
+        self.device = _device(device)
+        self.dim = structure.info()["dim"]
+        self.has_snch = bool(structure.info()["is_snch"])
+        self._structure = structure
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            opt = _lib.SceneOptions(leaf_prims=int(leaf_prims), packet_budget=int(packet_budget),
+                                    traversal=int(traversal), wave_front=int(wave_front))
+            _lib.check(_lib.lib().gq_scene_create_ex(structure.handle, C.byref(opt), C.byref(h)), "gq_scene_create")
+        self._h = h
+        # held by the object: module globals may already be gone when it is collected at exit
+        self._destroy = _lib.lib().gq_scene_destroy
+        self._sync = torch.cuda.synchronize
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._sync(self.device)
+            except Exception:
+                pass
+            self._destroy(h)
+            self._h = None
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 8)()
+        _lib.check(_lib.lib().gq_scene_info(self._h, a))
+        keys = ["dim", "has_snch", "device_bytes", "n_nodes", "n_wide_nodes", "n_prims", "n_silhouettes"]
+        return dict(zip(keys, list(a)[:7]))
+
     # -- queries -------------------------------------------------------------------------------
     def closest_point(self, x, method: str = "engine"):
         m = _lib.METHODS[method]
@@ -122,9 +221,7 @@ class Scene:
             out, r = self._alloc(n, ["valid", "p", "n", "uv", "d", "prim_id"])
             _lib.check(_lib.lib().gq_closest_point_bvh(self._h, m, _ptr(x), n, C.byref(r), self._stream()),
                        "closest_point_bvh")
-        if self.dim == 3:
-            return rec.ClosestPointRecord3D(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"])
-        return rec.ClosestPointRecord(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"])
+        return self._cp_record(out)
 
     def intersect(self, x, v, r_max=INF, method: str = "engine"):
         m = _lib.METHODS[method]
@@ -138,11 +235,7 @@ class Scene:
             out, r = self._alloc(n, ["valid", "p", "n", "uv", "d", "prim_id", "on_boundary"])
             _lib.check(_lib.lib().gq_intersect_bvh(self._h, m, _ptr(x), _ptr(v), _ptr(rt), rs, n, C.byref(r),
                                                    self._stream()), "intersect_bvh")
-        if self.dim == 3:
-            return rec.Intersection3D(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"],
-                                      out["on_boundary"])
-        return rec.Intersection(out["valid"], out["p"], out["n"], out["uv"], out["d"], out["prim_id"],
-                                out["on_boundary"])
+        return self._ray_record(out)
 
     def closest_silhouette(self, x, r_max=INF, method: str = "engine"):
         m = _lib.METHODS[method]
@@ -153,5 +246,4 @@ class Scene:
             out, r = self._alloc(n, ["valid", "p", "d", "prim_id"])
             _lib.check(_lib.lib().gq_closest_silhouette_snch(self._h, m, _ptr(x), _ptr(rt), rs, n, C.byref(r),
                                                              self._stream()), "closest_silhouette_snch")
-        cls = rec.ClosestSilhouettePointRecord3D if self.dim == 3 else rec.ClosestSilhouettePointRecord
-        return cls(out["valid"], out["p"], out["d"], out["prim_id"])
+        return self._sil_record(out)

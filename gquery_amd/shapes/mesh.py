@@ -6,7 +6,7 @@ same BVH twice, here one SNCH build provides both) and keeps them on the current
 from __future__ import annotations
 
 from .. import ext
-from ..scene import INF, Scene
+from ..scene import INF, BruteForce, Scene
 
 
 class Mesh:
@@ -17,6 +17,54 @@ class Mesh:
         self.vertices = self.snch._vertices
         self.indices = self.snch._indices
         self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal, wave_front)
+        self._device = self.scene.device
+        self._bf = None
+        self._normals = None
+        self._silhouettes = None
+
+    def _brute_force(self) -> BruteForce:
+        if self._bf is None:
+            self._bf = BruteForce(3, self.vertices, self.indices, self._device)
+        return self._bf
+
+    def configure_normal(self):
+        """Mesh.configure_normal (mesh.py:35-50): face normals normalize(cross(b-a, c-a)) and vertex
+        normals (normalised sums of the incident face normals), computed on the device."""
+        self._normals = self._brute_force().normals()
+        return self._normals
+
+    @property
+    def face_normals(self):
+        return (self._normals or self.configure_normal())[0]
+
+    @property
+    def normals(self):
+        return (self._normals or self.configure_normal())[1]
+
+    @property
+    def silhouettes(self):
+        """The mesh's silhouette edges (mesh.py:26-29): build_flat_silhouette_edges records."""
+        if self._silhouettes is None:
+            self._silhouettes = ext.build_flat_silhouette_edges(self.vertices, self.indices)
+        return self._silhouettes
+
+    def intersect(self, p, v, n=None, on_boundary=False, r_max=INF):
+        """Mesh.intersect (mesh.py:53-59): origins flagged on_boundary move by -1e-5 n, then the
+        brute-force ray query.  (The reference passes n and on_boundary on to intersect_baseline,
+        which does not take them; this runs what the method evidently means.)"""
+        return self._brute_force().intersect(p, v, r_max, n, on_boundary)
+
+    def intersect_baseline(self, p, v, r_max=INF):
+        """Mesh.intersect_baseline (mesh.py:61-85): every triangle for every ray."""
+        return self._brute_force().intersect(p, v, r_max)
+
+    def closest_point_baseline(self, x):
+        """Mesh.closest_point_baseline (mesh.py:92-118): every triangle for every point."""
+        return self._brute_force().closest_point(x)
+
+    def closest_silhouette_baseline(self, p, r_max=INF):
+        """Mesh.closest_silhouette_baseline (mesh.py:124-136): every silhouette edge."""
+        return self._brute_force().closest_silhouette(p, r_max)
 
     def intersect_bvh(self, p, v, r_max=INF, method: str = "engine"):
         """BVH3D.intersect (bvh3d.py:176-239) -> Intersection3D."""

@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 
 from .. import ext
-from ..scene import INF, Scene
+from ..scene import INF, BruteForce, Scene
 
 
 class Polyline:
@@ -21,6 +21,38 @@ class Polyline:
         self.vertices = self.snch._vertices
         self.indices = self.snch._indices
         self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal, wave_front)
+        self._device = self.scene.device
+        self._bf = None
+        self.configure_silhouette()
+
+    def configure_silhouette(self):
+        """Polyline.configure_silhouette (polyline.py:44-122): the silhouette vertices (vertices with
+        both neighbours, nearly collinear ones dropped) as (k, 10) float32 records a, b, c, prev,
+        vertex, next, index -- the reference's SilhouetteVertices layout."""
+        self.silhouettes = ext.configure_silhouette(self.vertices, self.indices)
+        return self.silhouettes
+
+    def _brute_force(self) -> BruteForce:
+        if self._bf is None:
+            self._bf = BruteForce(2, self.vertices, self.indices, self._device)
+        return self._bf
+
+    def intersect(self, p, v, n=None, on_boundary=False, r_max=INF):
+        """Polyline.intersect (polyline.py:125-131): origins flagged on_boundary move by -1e-5 n,
+        then the brute-force ray query."""
+        return self._brute_force().intersect(p, v, r_max, n, on_boundary)
+
+    def intersect_baseline(self, p, v, r_max=INF):
+        """Polyline.intersect_baseline (polyline.py:138-170): every segment for every ray."""
+        return self._brute_force().intersect(p, v, r_max)
+
+    def closest_point_baseline(self, p):
+        """Polyline.closest_point_baseline (polyline.py:176-209): every segment for every point."""
+        return self._brute_force().closest_point(p)
+
+    def closest_silhouette_baseline(self, x, r_max=INF):
+        """Polyline.closest_silhouette_baseline (polyline.py:212-223): every silhouette vertex."""
+        return self._brute_force().closest_silhouette(x, r_max)
 
     def intersect_bvh(self, p, v, r_max=INF, method: str = "engine"):
         """BVH.intersect (bvh.py:200-276) -> Intersection."""

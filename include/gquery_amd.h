@@ -165,6 +165,41 @@ int gq_batch_sizes(int n_jobs, const gq_job *jobs, int64_t *device_bytes, int64_
 int gq_batch_enqueue(int n_jobs, const gq_job *jobs, void *device_ws, int64_t device_bytes,
                      void *host_stage, int64_t host_bytes, void *stream);
 
+/* ---- Brute-force queries, normals and silhouette configuration ----
+ * A gq_shape is a polyline (dim 2, segments) or triangle mesh (dim 3) on the current device, for the
+ * reference's brute-force methods, which test every primitive or silhouette for every query.  Its
+ * silhouettes are those of Polyline.configure_silhouette (2D, polyline.py:44-122) or of
+ * build_flat_silhouette_edges (3D, the Mesh constructor, mesh.py:26-29); a mesh also gets its face
+ * and vertex normals (Mesh.configure_normal, mesh.py:35-50). */
+typedef struct gq_shape gq_shape;
+int gq_shape_create(int dim, const float *vertices, int64_t n_vertices, const int32_t *indices,
+                    int64_t n_prims, gq_shape **out);
+int gq_shape_destroy(gq_shape *shape);
+/* info[0..3] = dim, n_vertices, n_prims, n_silhouettes */
+int gq_shape_info(const gq_shape *shape, int64_t info[4]);
+/* Mesh.configure_normal (mesh.py:35-50): copies the face normals (n_prims x 3) and the vertex
+ * normals (n_vertices x 3) into device buffers (either may be NULL).  3D only. */
+int gq_shape_normals(const gq_shape *shape, float *face_normals, float *vertex_normals, void *stream);
+
+/* Polyline/Mesh.closest_point_baseline (polyline.py:176-209, mesh.py:92-118). */
+int gq_closest_point_baseline(gq_shape *shape, const float *x, int64_t n, const gq_records *out,
+                              void *stream);
+/* Polyline/Mesh.intersect_baseline (polyline.py:138-170, mesh.py:61-85).  With normal and
+ * on_boundary (n x dim floats, n bytes) it is Polyline/Mesh.intersect (polyline.py:125-131,
+ * mesh.py:53-59): origins on the boundary move by -1e-5 n first.  Either may be NULL only together. */
+int gq_intersect_baseline(gq_shape *shape, const float *x, const float *v, const float *normal,
+                          const uint8_t *on_boundary, const float *r_max, float r_max_scalar, int64_t n,
+                          const gq_records *out, void *stream);
+/* Polyline/Mesh.closest_silhouette_baseline (polyline.py:212-223, mesh.py:124-136). */
+int gq_closest_silhouette_baseline(gq_shape *shape, const float *x, const float *r_max,
+                                   float r_max_scalar, int64_t n, const gq_records *out, void *stream);
+
+/* Polyline.configure_silhouette (polyline.py:44-122) on the host: records of 10 floats (a, b, c,
+ * prev, vertex, next, first-appearance index) of the polyline's silhouette vertices.  Pass
+ * out = NULL to get the float count in *n_floats. */
+int gq_configure_silhouette(const float *vertices, int64_t n_vertices, const int32_t *edges,
+                            int64_t n_edges, float *out, int64_t *n_floats);
+
 /* ---- Engine layout inspection (no counterpart in the reference: exposes the host-built engine
  * arrays so tests can check them on machines without a GPU) ---- */
 

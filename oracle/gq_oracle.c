@@ -726,6 +726,166 @@ void or_bf_silhouette_f64(int D, const float *raw, int64_t ns, const float *x, i
     }
 }
 
+/* ---- Part 3 (FP32): brute-force baselines and mesh normals --------------------------------
+ *   or_normals        Mesh.configure_normal            gquery/shapes/mesh.py:35-50
+ *   or_cp_baseline    Polyline/Mesh.closest_point_baseline  polyline.py:176-209, mesh.py:92-118
+ *   or_ray_baseline   Polyline/Mesh.intersect_baseline (and .intersect's boundary offset)
+ *                                                      polyline.py:125-170, mesh.py:53-85
+ *   or_sil_baseline   Polyline/Mesh.closest_silhouette_baseline  polyline.py:212-223, mesh.py:124-136
+ * The loops visit primitives in index order and keep the first strictly nearer one. */
+
+void or_normals(const float *V, int64_t nv, const int32_t *F, int64_t nf, float *fn, float *vn) {
+    for (int64_t f = 0; f < nf; ++f) {
+        const float *a = V + 3 * (int64_t)F[3 * f], *b = V + 3 * (int64_t)F[3 * f + 1], *c = V + 3 * (int64_t)F[3 * f + 2];
+        float u[3], w[3];
+        for (int k = 0; k < 3; ++k) { u[k] = b[k] - a[k]; w[k] = c[k] - a[k]; }
+        cross3f(u, w, fn + 3 * f);
+        normalizef(3, fn + 3 * f);
+    }
+    /* dr.scatter_reduce(Add) over indices.x, then .y, then .z (mesh.py:44-49), into zeros */
+    memset(vn, 0, (size_t)nv * 3 * sizeof(float));
+    for (int slot = 0; slot < 3; ++slot)
+        for (int64_t f = 0; f < nf; ++f)
+            for (int k = 0; k < 3; ++k) vn[3 * (int64_t)F[3 * f + slot] + k] = vn[3 * (int64_t)F[3 * f + slot] + k] + fn[3 * f + k];
+    for (int64_t v = 0; v < nv; ++v) normalizef(3, vn + 3 * v);
+}
+
+void or_cp_baseline(int D, const float *V, const int32_t *F, int64_t nf, const float *fn, const float *x,
+                    int64_t n, or_out o) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        const float *p = x + D * i;
+        float d_min = INFINITY, pt[3], uv[2];
+        int idx = -1;
+        for (int64_t j = 0; j < nf; ++j) {
+            float d;
+            if (D == 2) {
+                const float *a = V + 2 * (int64_t)F[2 * j], *b = V + 2 * (int64_t)F[2 * j + 1];
+                float pa[2] = {p[0] - a[0], p[1] - a[1]}, ba[2] = {b[0] - a[0], b[1] - a[1]};
+                float h = clampf_(dot2f(pa, ba) / dot2f(ba, ba), 0.0f, 1.0f);
+                float q[2] = {pa[0] - ba[0] * h, pa[1] - ba[1] * h};
+                d = normf(2, q);
+            } else {
+                d = cp_triangle(p, V + 3 * (int64_t)F[3 * j], V + 3 * (int64_t)F[3 * j + 1], V + 3 * (int64_t)F[3 * j + 2], pt, uv);
+            }
+            if (d < d_min) { d_min = d; idx = (int)j; }
+        }
+        if (idx < 0) { zero_out(D, &o, i, 1); continue; }
+        if (D == 2) {
+            const float *a = V + 2 * (int64_t)F[2 * idx], *b = V + 2 * (int64_t)F[2 * idx + 1];
+            float pa[2] = {p[0] - a[0], p[1] - a[1]}, ba[2] = {b[0] - a[0], b[1] - a[1]};
+            float h = clampf_(dot2f(pa, ba) / dot2f(ba, ba), 0.0f, 1.0f);
+            float nn[2] = {ba[1], -ba[0]};
+            normalizef(2, nn);
+            if (o.p) for (int k = 0; k < 2; ++k) o.p[2 * i + k] = fmaf(b[k], h, fmaf(-a[k], h, a[k]));
+            if (o.n) { o.n[2 * i] = nn[0]; o.n[2 * i + 1] = nn[1]; }
+            if (o.uv) o.uv[i] = h;
+        } else {
+            cp_triangle(p, V + 3 * (int64_t)F[3 * idx], V + 3 * (int64_t)F[3 * idx + 1], V + 3 * (int64_t)F[3 * idx + 2], pt, uv);
+            if (o.p) memcpy(o.p + 3 * i, pt, 12);
+            if (o.n) memcpy(o.n + 3 * i, fn + 3 * (int64_t)idx, 12);
+            if (o.uv) memcpy(o.uv + 2 * i, uv, 8);
+        }
+        if (o.d) o.d[i] = d_min;
+        if (o.prim_id) o.prim_id[i] = idx;
+        if (o.valid) o.valid[i] = 1;
+    }
+}
+
+void or_ray_baseline(int D, const float *V, const int32_t *F, int64_t nf, const float *x, const float *vdir,
+                     const float *normal, const uint8_t *on_boundary, const float *r_max_arr, float r_max_scalar,
+                     int64_t n, or_out o) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        float p[3] = {0, 0, 0};
+        const float *v = vdir + D * i;
+        for (int k = 0; k < D; ++k) p[k] = x[D * i + k];
+        if (on_boundary && on_boundary[i])   /* p - 1e-5 * n (polyline.py:129, mesh.py:57) */
+            for (int k = 0; k < D; ++k) p[k] = p[k] - normal[D * i + k] * 1e-5f;
+        const float r_max = r_max_arr ? r_max_arr[i] : r_max_scalar;
+        float d_min = r_max, vv = 0, ww = 0;
+        int idx = -1;
+        for (int64_t j = 0; j < nf; ++j) {
+            float d;
+            if (D == 2) d = ray_seg(p, v, V + 2 * (int64_t)F[2 * j], V + 2 * (int64_t)F[2 * j + 1]);
+            else d = ray_tri(p, v, V + 3 * (int64_t)F[3 * j], V + 3 * (int64_t)F[3 * j + 1], V + 3 * (int64_t)F[3 * j + 2], &vv, &ww);
+            if (d < d_min) { d_min = d; idx = (int)j; }
+        }
+        if (idx < 0) { zero_out(D, &o, i, 1); continue; }
+        if (D == 2) {
+            const float *a = V + 2 * (int64_t)F[2 * idx], *b = V + 2 * (int64_t)F[2 * idx + 1];
+            float ab[2] = {b[0] - a[0], b[1] - a[1]};
+            normalizef(2, ab);
+            if (o.p) for (int k = 0; k < 2; ++k) o.p[2 * i + k] = p[k] + v[k] * d_min;
+            if (o.n) { o.n[2 * i] = ab[1]; o.n[2 * i + 1] = -ab[0]; }
+            if (o.uv) o.uv[i] = -1.0f;
+            if (o.d) o.d[i] = d_min;
+        } else {
+            const float *a = V + 3 * (int64_t)F[3 * idx], *b = V + 3 * (int64_t)F[3 * idx + 1], *c = V + 3 * (int64_t)F[3 * idx + 2];
+            float t = ray_tri(p, v, a, b, c, &vv, &ww), v1[3], v2[3], nn[3];
+            for (int k = 0; k < 3; ++k) { v1[k] = b[k] - a[k]; v2[k] = c[k] - a[k]; }
+            cross3f(v1, v2, nn);
+            normalizef(3, nn);
+            if (o.p) for (int k = 0; k < 3; ++k) o.p[3 * i + k] = a[k] + v1[k] * vv + v2[k] * ww;
+            if (o.n) memcpy(o.n + 3 * i, nn, 12);
+            if (o.uv) { o.uv[2 * i] = 1.0f - vv - ww; o.uv[2 * i + 1] = vv; }
+            if (o.d) o.d[i] = t;
+        }
+        if (o.prim_id) o.prim_id[i] = idx;
+        if (o.on_boundary) o.on_boundary[i] = 1;
+        if (o.valid) o.valid[i] = 1;
+    }
+}
+
+void or_sil_baseline(int D, const float *sils, int64_t ns, const float *x, const float *r_max_arr, float r_max_scalar,
+                     int64_t n, or_out o) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        const float *p = x + D * i;
+        const float r_max = r_max_arr ? r_max_arr[i] : r_max_scalar;
+        float d_min = r_max, best[3] = {0, 0, 0};
+        int id = -1;
+        for (int64_t j = 0; j < ns; ++j) {
+            if (D == 2) {
+                /* SilhouetteVertex.closest_silhouette (silhouette_vertex.py:39-53) */
+                const float *r = sils + 10 * j, *a = r, *b = r + 2, *c = r + 4;
+                float q[2] = {p[0] - b[0], p[1] - b[1]};
+                float d = normf(2, q);
+                if (((int)r[6] != -1) && ((int)r[8] != -1) && d < r_max) {
+                    float ba[2] = {b[0] - a[0], b[1] - a[1]}, xa[2] = {p[0] - a[0], p[1] - a[1]};
+                    float cb[2] = {c[0] - b[0], c[1] - b[1]}, xb[2] = {p[0] - b[0], p[1] - b[1]};
+                    if (cross2f(ba, xa) * cross2f(cb, xb) < 0.0f && d < d_min) {
+                        d_min = d; best[0] = b[0]; best[1] = b[1]; id = (int)r[9];
+                    }
+                }
+            } else {
+                /* SilhouetteEdge.closest_silhouette (silhouette_edge.py:49-64), flat edge records */
+                const float *r = sils + 19 * j, *c = r, *a = r + 3, *b = r + 6, *e = r + 9;
+                float pt[3];
+                float d = cp_segment(3, p, a, b, pt);
+                if (d < r_max) {
+                    int sil = ((int)r[12] == -1) || ((int)r[15] == -1);
+                    if (!sil) {
+                        float u[3], w[3], n0[3], n1[3], vd[3];
+                        for (int k = 0; k < 3; ++k) { u[k] = b[k] - a[k]; w[k] = c[k] - a[k]; }
+                        cross3f(u, w, n0); normalizef(3, n0);
+                        for (int k = 0; k < 3; ++k) { u[k] = e[k] - a[k]; w[k] = b[k] - a[k]; }
+                        cross3f(u, w, n1); normalizef(3, n1);
+                        for (int k = 0; k < 3; ++k) vd[k] = pt[k] - p[k];
+                        sil = dot3f(vd, n0) * dot3f(vd, n1) < 0.0f;
+                    }
+                    if (sil && d < d_min) { d_min = d; memcpy(best, pt, 12); id = (int)r[18]; }
+                }
+            }
+        }
+        if (id < 0) { zero_out(D, &o, i, 0); continue; }
+        if (o.valid) o.valid[i] = 1;
+        if (o.p) for (int k = 0; k < D; ++k) o.p[D * i + k] = best[k];
+        if (o.d) o.d[i] = d_min;
+        if (o.prim_id) o.prim_id[i] = id;
+    }
+}
+
 int or_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

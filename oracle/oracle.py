@@ -47,6 +47,10 @@ def lib():
         h.or_bf_intersect_f64.argtypes = [C.c_int, vp, vp, i64, vp, vp, i64, vp, vp, vp]
         h.or_bf_silhouette_f64.argtypes = [C.c_int, vp, i64, vp, i64, vp, vp, vp, vp]
         h.or_num_threads.restype = C.c_int
+        h.or_normals.argtypes = [vp, i64, vp, i64, vp, vp]
+        h.or_cp_baseline.argtypes = [C.c_int, vp, vp, i64, vp, vp, i64, _Out]
+        h.or_ray_baseline.argtypes = [C.c_int, vp, vp, i64, vp, vp, vp, vp, vp, f32, i64, _Out]
+        h.or_sil_baseline.argtypes = [C.c_int, vp, i64, vp, vp, f32, i64, _Out]
         _lib = h
     return _lib
 
@@ -139,3 +143,86 @@ def bf_silhouette(dim, raw_silhouette_data, x):
 
 def num_threads() -> int:
     return lib().or_num_threads()
+
+
+# ---- brute-force baselines, normals and silhouette configuration (FP32 restatements) ----------
+
+def normals(V, F):
+    """Mesh.configure_normal (mesh.py:35-50): (face normals, vertex normals)."""
+    V, F = _c(V, np.float32), _c(F, np.int32)
+    fn = np.zeros((F.shape[0], 3), np.float32)
+    vn = np.zeros((V.shape[0], 3), np.float32)
+    lib().or_normals(V.ctypes.data, V.shape[0], F.ctypes.data, F.shape[0], fn.ctypes.data, vn.ctypes.data)
+    return fn, vn
+
+
+def closest_point_baseline(dim, V, F, x, face_normals=None):
+    """Polyline/Mesh.closest_point_baseline (polyline.py:176-209, mesh.py:92-118)."""
+    V, F, x = _c(V, np.float32), _c(F, np.int32), _c(x, np.float32)
+    fn = _c(normals(V, F)[0] if (dim == 3 and face_normals is None) else
+            (face_normals if face_normals is not None else np.zeros((1, 3))), np.float32)
+    r, o = _records(x.shape[0], dim, "cp")
+    lib().or_cp_baseline(dim, V.ctypes.data, F.ctypes.data, F.shape[0], fn.ctypes.data, x.ctypes.data,
+                         x.shape[0], o)
+    return r
+
+
+def intersect_baseline(dim, V, F, x, v, r_max=float("inf"), normal=None, on_boundary=None):
+    """Polyline/Mesh.intersect_baseline (polyline.py:138-170, mesh.py:61-85); with normal and
+    on_boundary, Polyline/Mesh.intersect (polyline.py:125-131, mesh.py:53-59)."""
+    V, F, x, v = _c(V, np.float32), _c(F, np.int32), _c(x, np.float32), _c(v, np.float32)
+    r, o = _records(x.shape[0], dim, "ray")
+    rm = _c(r_max, np.float32) if np.ndim(r_max) > 0 else None
+    nb = _c(normal, np.float32) if normal is not None else None
+    ob = _c(on_boundary, np.uint8) if on_boundary is not None else None
+    lib().or_ray_baseline(dim, V.ctypes.data, F.ctypes.data, F.shape[0], x.ctypes.data, v.ctypes.data,
+                          None if nb is None else nb.ctypes.data, None if ob is None else ob.ctypes.data,
+                          None if rm is None else rm.ctypes.data, float(r_max) if rm is None else 0.0,
+                          x.shape[0], o)
+    return r
+
+
+def silhouette_baseline(dim, sils, x, r_max=float("inf")):
+    """Polyline/Mesh.closest_silhouette_baseline (polyline.py:212-223, mesh.py:124-136) over the
+    configure_silhouette records (2D, stride 10) or the flat silhouette edges (3D, stride 19)."""
+    x = _c(x, np.float32)
+    sils = _c(sils, np.float32).reshape(-1)
+    ns = sils.size // (10 if dim == 2 else 19)
+    r, o = _records(x.shape[0], dim, "sil")
+    rm = _c(r_max, np.float32) if np.ndim(r_max) > 0 else None
+    lib().or_sil_baseline(dim, sils.ctypes.data, ns, x.ctypes.data, None if rm is None else rm.ctypes.data,
+                          float(r_max) if rm is None else 0.0, x.shape[0], o)
+    return r
+
+
+def configure_silhouette(V, E):
+    """Polyline.configure_silhouette (polyline.py:44-122) restated with Python containers: records
+    of 10 floats (a, b, c, prev, vertex, next, first-appearance index), nearly collinear vertices
+    dropped (float32 unit edge directions, |dot| >= 1 - 1e-5)."""
+    V = np.asarray(V, np.float32)
+    nbr = {}
+    for a, b in np.asarray(E, np.int64):
+        a, b = int(a), int(b)
+        for u in (a, b):
+            if u not in nbr:
+                nbr[u] = [-1, u, -1]
+        nbr[a][2] = b
+        nbr[b][0] = a
+    def dot(a, b):   # fma(a.y, b.y, a.x * b.x): the product of two float32 is exact in float64
+        return np.float32(np.float64(a[1]) * np.float64(b[1]) + np.float64(np.float32(a[0] * b[0])))
+
+    out = []
+    one = np.float32(1.0)
+    for i, u in enumerate(nbr):
+        p, _, q = nbr[u]
+        if p == -1 or q == -1:
+            continue
+        a, b, c = V[p], V[u], V[q]
+        e1, e2 = (b - a).astype(np.float32), (c - b).astype(np.float32)
+        e1 = e1 * (one / np.sqrt(dot(e1, e1)))
+        e2 = e2 * (one / np.sqrt(dot(e2, e2)))
+        col = np.abs(dot(e1, e2))
+        if not col < np.float32(1.0 - 1e-5):
+            continue
+        out.append([a[0], a[1], b[0], b[1], c[0], c[1], p, u, q, i])
+    return np.asarray(out, np.float32).reshape(-1, 10)
