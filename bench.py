@@ -296,13 +296,27 @@ def served(args, loads=(0.25, 0.5, 0.9)):
     scene_specs, req_specs = synth.served_workload(n_requests=args.requests, scale=args.scale)
     t0 = time.perf_counter()
 
-    def make(sp):
-        V, F = synth.scene_geometry(sp)
-        torch.cuda.set_device(0)
-        return (Mesh if sp["dim"] == 3 else Polyline)(V, F, device="cuda:0")
+    cache = Path(args.scene_cache) if args.scene_cache else None
 
+    def make(k_sp):
+        k, sp = k_sp
+        torch.cuda.set_device(0)
+        cls = Mesh if sp["dim"] == 3 else Polyline
+        f = cache / f"served_{k}_{sp['dim']}d_{sp['prims']}.gqs" if cache else None
+        if f is not None and f.exists():
+            loaded.append(k)
+            return cls.load(f, device="cuda:0")
+        V, F = synth.scene_geometry(sp)
+        shape = cls(V, F, device="cuda:0")
+        if f is not None:
+            shape.save(f)
+        return shape
+
+    loaded = []
+    if cache:
+        cache.mkdir(parents=True, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=8) as ex:
-        shapes = list(ex.map(make, scene_specs))
+        shapes = list(ex.map(make, enumerate(scene_specs)))
     inputs = []
     for q in req_specs:
         dim = scene_specs[q["scene"]]["dim"]
@@ -311,8 +325,10 @@ def served(args, loads=(0.25, 0.5, 0.9)):
         inputs.append((x, v))
     torch.cuda.synchronize()
     n_q = sum(q["n"] for q in req_specs)
-    log(f"served: {len(shapes)} scenes ({sum(sp['prims'] for sp in scene_specs) / 1e6:.2f}M primitives) built in "
-        f"{time.perf_counter() - t0:.0f}s; {len(req_specs)} requests, {n_q / 1e6:.2f}M queries per run")
+    setup_s = time.perf_counter() - t0
+    log(f"served: {len(shapes)} scenes ({sum(sp['prims'] for sp in scene_specs) / 1e6:.2f}M primitives) "
+        f"set up in {setup_s:.1f}s ({len(loaded)} loaded from scene files); "
+        f"{len(req_specs)} requests, {n_q / 1e6:.2f}M queries per run")
     slots = (2, 1)
     executor = GpuExecutor(shapes, slots=slots, method=args.method if args.method != "plain" else "engine")
     budget = int(torch.cuda.mem_get_info()[0] * 0.5)
@@ -352,7 +368,7 @@ def served(args, loads=(0.25, 0.5, 0.9)):
             "data": "synthetic: seeded scenes and Poisson request stream",
             "config": {"workload": f"served: {len(shapes)} scenes, {len(req_specs)} requests per load, "
                                    f"{n_q} queries, memory budget {budget} B, stream slots {slots} (cp+ray, sil)",
-                       "scale": args.scale},
+                       "scale": args.scale, "scene_setup_s": setup_s, "scenes_loaded": len(loaded)},
             "served": out}
 
 
@@ -395,6 +411,8 @@ def main():
                     help="serving loop: 16 resident scenes, Poisson request stream at 25/50/90%% of the "
                          "saturated rate, p50/p99 latency")
     ap.add_argument("--requests", type=int, default=400, help="--served: requests per load level")
+    ap.add_argument("--scene-cache", default="",
+                    help="--served: directory of scene files (loaded if present, else built and written)")
     ap.add_argument("--tune", action="store_true",
                     help="autotune leaf size / packet budget / silhouette traversal for the scene and "
                          "query kind given by --shape/--subdiv/--segments/--query, then time the best")

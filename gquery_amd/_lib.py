@@ -98,6 +98,9 @@ SIGNATURES = {
     "gq_intersect_baseline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_closest_silhouette_baseline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_configure_silhouette": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, _i64p]),
+    "gq_structure_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "gq_structure_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "gq_structure_geometry": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.c_void_p, _i64p]),
     "gq_profile_begin": (C.c_int, []),
     "gq_profile_end": (C.c_int, [C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
 }

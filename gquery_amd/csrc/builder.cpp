@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <unordered_map>
 #include <unordered_set>
@@ -442,6 +443,8 @@ bool build_impl(const float *V, int64_t nv, const int32_t *I, int64_t np, int ma
     out = HostStructure();
     out.dim = D;
     out.is_snch = with_snch;
+    out.vertices.assign(V, V + (size_t)nv * D);
+    out.indices.assign(I, I + (size_t)np * D);
     out.n_nodes = (int64_t)b.nodes.size();
     out.n_prims = np;
     out.max_depth = b.max_depth;
@@ -624,6 +627,104 @@ void vertex_faces(const int32_t *F, int64_t nf, int64_t nv, std::vector<int32_t>
     std::vector<int32_t> fill(start.begin(), start.end() - 1);
     for (int slot = 0; slot < 3; ++slot)
         for (int64_t f = 0; f < nf; ++f) list[(size_t)fill[(size_t)F[3 * f + slot]]++] = (int32_t)f;
+}
+
+namespace {
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void *p, size_t n) {
+        const uint8_t *b = (const uint8_t *)p;
+        for (size_t k = 0; k < n; ++k) { h ^= b[k]; h *= 1099511628211ull; }
+    }
+};
+const char kSceneMagic[8] = {'G', 'Q', 'S', 'C', 'E', 'N', 'E', '1'};
+} // namespace
+
+bool save_structure(const HostStructure &h, const char *path, std::string &err) {
+    std::FILE *f = path ? std::fopen(path, "wb") : nullptr;
+    if (!f) { err = std::string("cannot open for writing: ") + (path ? path : "(null)"); return false; }
+    Fnv sum;
+    bool ok = true;
+    auto put = [&](const void *p, size_t n) {
+        if (ok && n && std::fwrite(p, 1, n, f) != n) ok = false;
+        sum.add(p, n);
+    };
+    auto put_vec = [&](const auto &v) {
+        const int64_t n = (int64_t)v.size();
+        put(&n, 8);
+        put(v.data(), v.size() * sizeof(v[0]));
+    };
+    const int64_t head[8] = {h.dim, h.is_snch ? 1 : 0, h.n_nodes, h.n_prims, h.n_silhouettes, h.n_raw_silhouettes,
+                             h.max_depth, 0};
+    put(kSceneMagic, 8);
+    put(head, sizeof head);
+    put_vec(h.bvh_node_data);
+    put_vec(h.snch_node_data);
+    put_vec(h.primitive_data);
+    put_vec(h.silhouette_data);
+    put_vec(h.raw_silhouette_data);
+    put_vec(h.ordered_prim_index);
+    put_vec(h.vertices);
+    put_vec(h.indices);
+    const uint64_t c = sum.h;
+    if (ok && std::fwrite(&c, 1, 8, f) != 8) ok = false;
+    if (std::fclose(f) != 0) ok = false;
+    if (!ok) err = std::string("write failed: ") + path;
+    return ok;
+}
+
+bool load_structure(const char *path, HostStructure &h, std::string &err) {
+    std::FILE *f = path ? std::fopen(path, "rb") : nullptr;
+    if (!f) { err = std::string("cannot open: ") + (path ? path : "(null)"); return false; }
+    Fnv sum;
+    bool ok = true;
+    auto get = [&](void *p, size_t n) {
+        if (ok && n && std::fread(p, 1, n, f) != n) ok = false;
+        if (ok) sum.add(p, n);
+    };
+    auto get_vec = [&](auto &v, int64_t want) {
+        int64_t n = -1;
+        get(&n, 8);
+        if (!ok || n < 0 || (want >= 0 && n != want) || n > ((int64_t)1 << 40)) { ok = false; return; }
+        v.resize((size_t)n);
+        get(v.data(), v.size() * sizeof(v[0]));
+    };
+    char magic[8] = {0};
+    int64_t head[8] = {0};
+    get(magic, 8);
+    get(head, sizeof head);
+    HostStructure s;
+    if (ok && std::memcmp(magic, kSceneMagic, 8) != 0) { ok = false; }
+    const int D = (int)head[0];
+    if (ok && (D != 2 && D != 3)) ok = false;
+    if (ok) {
+        s.dim = D;
+        s.is_snch = head[1] != 0;
+        s.n_nodes = head[2];
+        s.n_prims = head[3];
+        s.n_silhouettes = head[4];
+        s.n_raw_silhouettes = head[5];
+        s.max_depth = (int32_t)head[6];
+        if (s.n_nodes < 1 || s.n_prims < 1 || s.n_silhouettes < 0 || s.n_raw_silhouettes < 0) ok = false;
+    }
+    if (ok) {
+        get_vec(s.bvh_node_data, s.n_nodes * kBvhNodeStride[D]);
+        get_vec(s.snch_node_data, s.is_snch ? s.n_nodes * kSnchNodeStride[D] : 0);
+        get_vec(s.primitive_data, s.n_prims * kPrimStride[D]);
+        get_vec(s.silhouette_data, s.is_snch ? s.n_silhouettes * kSilStride[D] : 0);
+        get_vec(s.raw_silhouette_data, s.is_snch ? s.n_raw_silhouettes * kSilStride[D] : 0);
+        get_vec(s.ordered_prim_index, s.n_prims);
+        get_vec(s.vertices, -1);
+        get_vec(s.indices, s.n_prims * D);
+    }
+    uint64_t want = sum.h, c = 0;
+    if (ok && std::fread(&c, 1, 8, f) != 8) ok = false;
+    std::fclose(f);
+    if (ok && c != want) { err = std::string("checksum mismatch: ") + path; return false; }
+    if (ok && s.vertices.size() % (size_t)D != 0) ok = false;
+    if (!ok) { err = std::string("not a valid scene file: ") + path; return false; }
+    h = std::move(s);
+    return true;
 }
 
 bool build_flat_silhouette_edges(const float *V, int64_t nv, const int32_t *F, int64_t nf,

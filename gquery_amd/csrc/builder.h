@@ -35,7 +35,17 @@ struct HostStructure {
     std::vector<float> silhouette_data; // stride kSilStride[dim], leaf order (SNCH only)
     std::vector<float> raw_silhouette_data;
     std::vector<int32_t> ordered_prim_index;  // original primitive id of each ordered slot
+    std::vector<float> vertices;              // the input geometry (n_vertices x dim)
+    std::vector<int32_t> indices;             // n_prims x dim
 };
+
+// Scene file: the whole HostStructure (flat arrays, counts, input geometry) in one binary file, so
+// a serving process loads a scene without running the builder.  Little-endian records: magic
+// "GQSCENE1", a header of counts, then each array as (int64 count, data); an FNV-1a 64-bit
+// checksum of everything before it closes the file.  load_structure checks the magic, the counts
+// against the record strides, and the checksum.
+bool save_structure(const HostStructure &h, const char *path, std::string &err);
+bool load_structure(const char *path, HostStructure &h, std::string &err);
 
 // Builds the BVH (and, if with_snch, the SNCH) of a polyline (dim 2: indices are segments) or a
 // triangle mesh (dim 3).  Returns false and fills err on invalid input.

@@ -187,6 +187,41 @@ int gq_structure_raw_silhouette_data(const gq_structure *s, float *out, int64_t 
     return copy_out(s->h.raw_silhouette_data, out, n);
 }
 
+int gq_structure_save(const gq_structure *s, const char *path) {
+    if (!s || !path) return fail(GQ_ERR_INVALID, "null argument");
+    std::string err;
+    if (!save_structure(s->h, path, err)) return fail(GQ_ERR_INVALID, err);
+    return GQ_OK;
+}
+
+int gq_structure_load(const char *path, gq_structure **out) {
+    if (!path || !out) return fail(GQ_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<gq_structure> s(new (std::nothrow) gq_structure());
+    if (!s) return fail(GQ_ERR_OOM, "host allocation failed");
+    std::string err;
+    try {
+        if (!load_structure(path, s->h, err)) return fail(GQ_ERR_INVALID, err);
+    } catch (const std::bad_alloc &) {
+        return fail(GQ_ERR_OOM, "host allocation failed while loading");
+    }
+    *out = s.release();
+    return GQ_OK;
+}
+
+int gq_structure_geometry(const gq_structure *s, float *vertices, int64_t *n_floats, int32_t *indices,
+                          int64_t *n_ints) {
+    if (!s || !n_floats || !n_ints) return fail(GQ_ERR_INVALID, "null argument");
+    const HostStructure &h = s->h;
+    if ((vertices && *n_floats < (int64_t)h.vertices.size()) || (indices && *n_ints < (int64_t)h.indices.size()))
+        return fail(GQ_ERR_INVALID, "output buffer too small");
+    if (vertices) std::memcpy(vertices, h.vertices.data(), h.vertices.size() * sizeof(float));
+    if (indices) std::memcpy(indices, h.indices.data(), h.indices.size() * sizeof(int32_t));
+    *n_floats = (int64_t)h.vertices.size();
+    *n_ints = (int64_t)h.indices.size();
+    return GQ_OK;
+}
+
 int gq_build_flat_silhouette_edges(const float *vertices, int64_t n_vertices, const int32_t *faces,
                                    int64_t n_faces, float *out, int64_t *n_floats) {
     std::vector<float> v;

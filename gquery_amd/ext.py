@@ -80,6 +80,10 @@ class _Structure:
     def primitive_data(self) -> np.ndarray:
         return self._fetch("gq_structure_primitive_data")
 
+    def save(self, path) -> None:
+        """Writes the structure (flat arrays, counts, input geometry) to a scene file."""
+        _lib.check(_lib.lib().gq_structure_save(self._h, str(path).encode()), "gq_structure_save")
+
     def engine_layout(self, which: int) -> np.ndarray:
         """Raw bytes of one engine-layout array built on the host (see gq_structure_engine_layout)."""
         L = _lib.lib()
@@ -119,6 +123,32 @@ class SNCH3D(_SnchMixin, _Structure):
     """gquery_ext.SNCH3D: BVH + normal cones + silhouette edges."""
     _dim = 3
     _snch = True
+
+
+def load(path) -> _Structure:
+    """Reads a scene file written by save(): a BVH, BVH3D, SNCH or SNCH3D without running the
+    builder (raises ValueError on a file that is not a complete, uncorrupted scene file)."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    _lib.check(L.gq_structure_load(str(path).encode(), C.byref(h)), "gq_structure_load")
+    a = (C.c_int64 * 8)()
+    rc = L.gq_structure_info(h, a)
+    if rc != _lib.GQ_OK:
+        L.gq_structure_free(h)
+        _lib.check(rc, "gq_structure_info")
+    dim, snch = int(a[0]), bool(a[1])
+    cls = {(2, False): BVH, (3, False): BVH3D, (2, True): SNCH, (3, True): SNCH3D}[(dim, snch)]
+    obj = cls.__new__(cls)
+    obj._h = h
+    obj._free = L.gq_structure_free
+    nf, ni = C.c_int64(0), C.c_int64(0)
+    _lib.check(L.gq_structure_geometry(h, None, C.byref(nf), None, C.byref(ni)), "gq_structure_geometry")
+    v = np.empty(nf.value, np.float32)
+    f = np.empty(ni.value, np.int32)
+    _lib.check(L.gq_structure_geometry(h, v.ctypes.data, C.byref(nf), f.ctypes.data, C.byref(ni)),
+               "gq_structure_geometry")
+    obj._vertices, obj._indices = v.reshape(-1, dim), f.reshape(-1, dim)
+    return obj
 
 
 def build_flat_silhouette_edges(vertices, faces) -> np.ndarray:

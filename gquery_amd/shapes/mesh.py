@@ -11,8 +11,12 @@ from ..scene import INF, BruteForce, Scene
 
 class Mesh:
     def __init__(self, vertices, indices, device=None, max_prims_in_node: int = 10, leaf_prims: int = 0,
-                 packet_budget: int = 0, traversal: int = 0, wave_front: int = 0):
-        self.snch = ext.SNCH3D(vertices, indices, max_prims_in_node)
+                 packet_budget: int = 0, traversal: int = 0, wave_front: int = 0, structure=None):
+        # structure: an ext.SNCH3D already built or loaded from a scene file (then vertices and
+        # indices may be None)
+        self.snch = structure if structure is not None else ext.SNCH3D(vertices, indices, max_prims_in_node)
+        if not isinstance(self.snch, ext.SNCH3D):
+            raise ValueError("structure must be an ext.SNCH3D")
         self.bvh = self.snch
         self.vertices = self.snch._vertices
         self.indices = self.snch._indices
@@ -77,3 +81,13 @@ class Mesh:
     def closest_silhouette_snch(self, p, r_max=INF, method: str = "engine"):
         """SNCH3D.closest_silhouette (snch3d.py:143-203) -> ClosestSilhouettePointRecord3D."""
         return self.scene.closest_silhouette(p, r_max, method)
+
+    def save(self, path) -> None:
+        """Writes the mesh's trees and geometry to a scene file (Mesh.load reads it back
+        without running the builder)."""
+        self.snch.save(path)
+
+    @classmethod
+    def load(cls, path, device=None, **options) -> "Mesh":
+        """A Mesh from a scene file written by save(); options as for the constructor."""
+        return cls(None, None, device=device, structure=ext.load(path), **options)

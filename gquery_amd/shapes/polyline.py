@@ -14,9 +14,12 @@ from ..scene import INF, BruteForce, Scene
 
 class Polyline:
     def __init__(self, vertices, indices, device=None, max_prims_in_node: int = 10, leaf_prims: int = 0,
-                 packet_budget: int = 0, traversal: int = 0, wave_front: int = 0):
-        # One SNCH build serves both queries: its BVH part is the same tree BVH() builds.
-        self.snch = ext.SNCH(vertices, indices, max_prims_in_node)
+                 packet_budget: int = 0, traversal: int = 0, wave_front: int = 0, structure=None):
+        # structure: an ext.SNCH already built or loaded from a scene file (then vertices and
+        # indices may be None)
+        self.snch = structure if structure is not None else ext.SNCH(vertices, indices, max_prims_in_node)
+        if not isinstance(self.snch, ext.SNCH):
+            raise ValueError("structure must be an ext.SNCH")
         self.bvh = self.snch
         self.vertices = self.snch._vertices
         self.indices = self.snch._indices
@@ -65,3 +68,13 @@ class Polyline:
     def closest_silhouette_snch(self, x, r_max=INF, method: str = "engine"):
         """SNCH.closest_silhouette (snch.py:106-164) -> ClosestSilhouettePointRecord."""
         return self.scene.closest_silhouette(x, r_max, method)
+
+    def save(self, path) -> None:
+        """Writes the polyline's trees and geometry to a scene file (Polyline.load reads it back
+        without running the builder)."""
+        self.snch.save(path)
+
+    @classmethod
+    def load(cls, path, device=None, **options) -> "Polyline":
+        """A Polyline from a scene file written by save(); options as for the constructor."""
+        return cls(None, None, device=device, structure=ext.load(path), **options)

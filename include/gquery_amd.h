@@ -81,6 +81,17 @@ int gq_structure_primitive_data(const gq_structure *s, float *out, int64_t *n_fl
 int gq_structure_silhouette_data(const gq_structure *s, float *out, int64_t *n_floats);
 int gq_structure_raw_silhouette_data(const gq_structure *s, float *out, int64_t *n_floats);
 
+/* Scene files (no counterpart in the reference, which rebuilds the trees in every process): the
+ * whole structure -- flat arrays, counts and the input geometry -- in one binary file with a
+ * checksum, so a serving process loads a scene without running the builder.  gq_structure_load
+ * rejects a file whose magic, sizes or checksum do not match (GQ_ERR_INVALID). */
+int gq_structure_save(const gq_structure *s, const char *path);
+int gq_structure_load(const char *path, gq_structure **out);
+/* The input geometry of a structure: n_vertices x dim floats and n_prims x dim indices.  Pass
+ * NULL outputs to get the counts in *n_floats and *n_ints. */
+int gq_structure_geometry(const gq_structure *s, float *vertices, int64_t *n_floats, int32_t *indices,
+                          int64_t *n_ints);
+
 /* gquery_ext.build_flat_silhouette_edges (src/shapes/silhouette_edge.cpp:62-71). */
 int gq_build_flat_silhouette_edges(const float *vertices, int64_t n_vertices, const int32_t *faces,
                                    int64_t n_faces, float *out, int64_t *n_floats);
