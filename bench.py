@@ -329,7 +329,7 @@ def served(args, loads=(0.25, 0.5, 0.9)):
     log(f"served: {len(shapes)} scenes ({sum(sp['prims'] for sp in scene_specs) / 1e6:.2f}M primitives) "
         f"set up in {setup_s:.1f}s ({len(loaded)} loaded from scene files); "
         f"{len(req_specs)} requests, {n_q / 1e6:.2f}M queries per run")
-    slots = (2, 1)
+    slots = tuple(int(a) for a in args.slots.split(","))
     executor = GpuExecutor(shapes, slots=slots, method=args.method if args.method != "plain" else "engine")
     budget = int(torch.cuda.mem_get_info()[0] * 0.5)
 
@@ -337,7 +337,7 @@ def served(args, loads=(0.25, 0.5, 0.9)):
         reqs = [Request(rid=i, scene=q["scene"], kind=q["kind"], x=inputs[i][0], v=inputs[i][1])
                 for i, q in enumerate(req_specs)]
         launches0 = executor.launches
-        sched = Scheduler(executor, memory_budget=budget, slots=slots)
+        sched = Scheduler(executor, memory_budget=budget, slots=slots, max_batch_queries=args.max_batch)
         run_open_loop(sched, reqs, arrivals)
         torch.cuda.synchronize()
         lat = [r.latency * 1e3 for r in reqs]
@@ -411,6 +411,8 @@ def main():
                     help="serving loop: 16 resident scenes, Poisson request stream at 25/50/90%% of the "
                          "saturated rate, p50/p99 latency")
     ap.add_argument("--requests", type=int, default=400, help="--served: requests per load level")
+    ap.add_argument("--slots", default="2,1", help="--served: stream slots of the (cp+ray, silhouette) classes")
+    ap.add_argument("--max-batch", type=int, default=1 << 20, help="--served: queries per coalesced launch at most")
     ap.add_argument("--scene-cache", default="",
                     help="--served: directory of scene files (loaded if present, else built and written)")
     ap.add_argument("--tune", action="store_true",
