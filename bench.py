@@ -230,6 +230,14 @@ def batched(args):
     n_q = sum(sp["queries"] for sp in specs)
     log(f"{len(specs)} scenes, {n_prims / 1e6:.2f}M primitives, {n_q / 1e6:.2f}M queries, built in "
         f"{time.perf_counter() - t0:.0f}s")
+    tuned = None
+    if args.tune:
+        from gquery_amd import tune
+        t1 = time.perf_counter()
+        tuned = tune.tune_jobs(jobs, log=log)
+        for (shape, _, _, _), opt in zip(jobs, tuned):
+            shape.set_options(**opt)
+        log(f"autotuned {len(set(map(str, tuned)))} option sets in {time.perf_counter() - t1:.0f}s")
     stream = torch.cuda.current_stream()
 
     def single_all():
@@ -276,7 +284,8 @@ def batched(args):
                                   f"(cp {by_kind['cp']}, ray {by_kind['ray']}, sil {by_kind['sil']}), scale {args.scale}",
                       "method": method, "parallelism": "single GPU"},
            "sequential_single_calls_ms": ms_seq, "graph_speedup_vs_sequential": ms_seq / ms_graph,
-           "device_workspace_bytes": b.device_bytes}
+           "device_workspace_bytes": b.device_bytes,
+           "autotuned_options": None if tuned is None else sorted({json.dumps(o, sort_keys=True) for o in tuned})}
     print(json.dumps(out))
 
 
@@ -417,10 +426,14 @@ def main():
                     help="--served: directory of scene files (loaded if present, else built and written)")
     ap.add_argument("--tune", action="store_true",
                     help="autotune leaf size / packet budget / silhouette traversal for the scene and "
-                         "query kind given by --shape/--subdiv/--segments/--query, then time the best")
+                         "query kind given by --shape/--subdiv/--segments/--query, then time the best; "
+                         "with --batched: tune each group of similar jobs and run the batch with the choices")
     args = ap.parse_args()
     if args.served:
         print(json.dumps(served(args)))
+        return
+    if args.batched:
+        batched(args)
         return
     if args.tune:
         import torch
@@ -434,9 +447,7 @@ def main():
         print(json.dumps({"shape": args.shape, "query": args.query, "n": args.n, "best": res["options"],
                           "ms": res["ms"], "trials": res["trials"]}))
         return
-    if args.batched:
-        batched(args)
-        return
+
     if args.accuracy:
         accuracy(args)
         return

@@ -82,6 +82,10 @@ class Mesh:
         """SNCH3D.closest_silhouette (snch3d.py:143-203) -> ClosestSilhouettePointRecord3D."""
         return self.scene.closest_silhouette(p, r_max, method)
 
+    def set_options(self, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0, wave_front: int = 0):
+        """Rebuilds the device scene with other engine options (the autotuner's choice)."""
+        self.scene = Scene(self.snch, self._device, leaf_prims, packet_budget, traversal, wave_front)
+
     def save(self, path) -> None:
         """Writes the mesh's trees and geometry to a scene file (Mesh.load reads it back
         without running the builder)."""

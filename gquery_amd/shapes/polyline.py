@@ -69,6 +69,10 @@ class Polyline:
         """SNCH.closest_silhouette (snch.py:106-164) -> ClosestSilhouettePointRecord."""
         return self.scene.closest_silhouette(x, r_max, method)
 
+    def set_options(self, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0, wave_front: int = 0):
+        """Rebuilds the device scene with other engine options (the autotuner's choice)."""
+        self.scene = Scene(self.snch, self._device, leaf_prims, packet_budget, traversal, wave_front)
+
     def save(self, path) -> None:
         """Writes the polyline's trees and geometry to a scene file (Polyline.load reads it back
         without running the builder)."""

@@ -111,3 +111,44 @@ def tune_shape(V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1
         if not fast:
             opts["packet_budget"] = s["packet_budget"]
     return {"options": opts, "ms": ms, "trials": trials}
+
+
+
+def tune_jobs(jobs, leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, method: str = "engine", log=None):
+    """Options for a batch of jobs [(shape, kind, x, v), ...] (Mesh / Polyline shapes), timed as they
+    will run: jobs are grouped by (dimension, query kind, primitive count within a factor of 4), and
+    for every leaf size each group's jobs run as one gq_batch_enqueue batch with every packet
+    budget / traversal candidate as per-job overrides (one device scene per shape and leaf size).
+    Returns one options dict per job (leaf_prims, packet_budget, traversal)."""
+    from .batch import Batch
+    from .scene import Scene
+    groups = {}
+    for k, (shape, kind, x, v) in enumerate(jobs):
+        n_prims = int(shape.indices.shape[0])
+        key = (shape.scene.dim, kind, int(np.log2(max(n_prims, 1)) // 2))
+        groups.setdefault(key, []).append(k)
+    best = {key: (float("inf"), None) for key in groups}
+    for leaf in leaves:
+        scenes = {}
+        for k, (shape, _, _, _) in enumerate(jobs):
+            scenes[k] = Scene(shape.snch, shape.scene.device, leaf)
+        for key, ks in groups.items():
+            kind = key[1]
+            for budget, trav in itertools.product(budgets, traversals):
+                if (kind != "sil" and trav == 2) or ((trav == 2 or kind == "ray") and budget != budgets[0]):
+                    continue    # traversal only matters for silhouettes; lane-per-query kernels ignore the budget
+                b = Batch([dict(shape=scenes[k], kind=kind, x=jobs[k][2], v=jobs[k][3], packet_budget=budget,
+                                traversal=trav) for k in ks], method=method)
+                ms = _time(b.run)
+                if ms < best[key][0]:
+                    best[key] = (ms, {"leaf_prims": leaf, "packet_budget": budget, "traversal": trav})
+                del b
+        del scenes
+    out = [None] * len(jobs)
+    for key, ks in sorted(groups.items()):
+        if log:
+            log(f"tune group dim={key[0]} kind={key[1]} prims~4^{key[2]} ({len(ks)} jobs): {best[key][1]} "
+                f"{best[key][0]:.3f} ms")
+        for k in ks:
+            out[k] = dict(best[key][1])
+    return out

@@ -33,3 +33,23 @@ def test_tuned_options_give_the_default_records(gpu, dim):
             rec.append({f: getattr(r, f).cpu().numpy() for f in FIELDS})
         for f in FIELDS:
             assert np.array_equal(rec[0][f], rec[1][f]), (q, f)
+
+
+def test_tune_jobs_groups_and_keeps_results(gpu):
+    from gquery_amd.batch import Batch
+    shapes = [Polyline(*synth.star_polyline(2000, seed=s), device="cuda:0") for s in (1, 2)]
+    shapes.append(Mesh(*synth.displaced_icosphere(3, seed=3), device="cuda:0"))
+    jobs = []
+    for k, (shape, kind) in enumerate(zip(shapes, ("cp", "cp", "sil"))):
+        dim = shape.scene.dim
+        jobs.append((shape, kind, torch.from_numpy(synth.uniform_points(9000, dim, seed=70 + k)).cuda(), None))
+    ref = [j[0].closest_point_bvh(j[2]) if j[1] == "cp" else j[0].closest_silhouette_snch(j[2]) for j in jobs]
+    opts = tune.tune_jobs(jobs, leaves=(2, 6), budgets=(500, 1000), traversals=(1, 2))
+    assert opts[0] == opts[1]                       # same group: same dimension, kind and size class
+    for (shape, _, _, _), o in zip(jobs, opts):
+        shape.set_options(**o)
+    got = Batch(jobs).run()
+    torch.cuda.synchronize()
+    for a, b in zip(ref, got):
+        for f in ("valid", "d", "prim_id"):
+            assert torch.equal(getattr(a, f), getattr(b, f))
