@@ -558,6 +558,29 @@ def main():
         cpu = cpu_baseline(mesh, x_host)
         log(f"cpu baseline: {cpu['value']:.3f} Mq/s ({cpu['sample']})")
 
+    # end to end from host memory: pinned queries in, query, the records back to pinned host memory
+    xh = torch.from_numpy(x_host).pin_memory()
+    pinned = {}
+    with torch.no_grad():
+        r0 = mesh.closest_point_bvh(x, method=method)
+        for f in ("valid", "p", "n", "uv", "d", "prim_id"):
+            t = getattr(r0, f)
+            pinned[f] = torch.empty(t.shape, dtype=t.dtype).pin_memory()
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(PLAIN_STEPS):
+        xd = xh.to("cuda:0", non_blocking=True)
+        r0 = mesh.closest_point_bvh(xd, method=method)
+        for f, t in pinned.items():
+            t.copy_(getattr(r0, f), non_blocking=True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    host_ms = ev0.elapsed_time(ev1) / PLAIN_STEPS
+    host_rate = {"ms_per_step": host_ms, "value": nq / (host_ms * 1e-3) / 1e6,
+                 "bytes": int(x_host.nbytes + sum(t.numel() * t.element_size() for t in pinned.values()))}
+    log(f"from host memory (pinned, PCIe both ways): {host_ms:.2f} ms/step, {host_rate['value']:.1f} Mq/s")
+    del xh, pinned, r0
+
     plain = None
     if not args.no_plain and method != "plain":
         # the plain FP32 kernels (reference layout, lane per query) on the same resident batch
@@ -600,6 +623,7 @@ def main():
         "cpu_baseline": cpu,
         "vs_plain": plain["speedup"] if plain else None,
         "plain": plain,
+        "host_memory": host_rate,
         "served": srv,
         "max_rel_err_vs_oracle": max_err,
     }
