@@ -24,7 +24,7 @@ METHODS = {"plain": METHOD_PLAIN, "engine_fp32": METHOD_ENGINE_FP32, "engine": M
 
 class SceneOptions(C.Structure):
     _fields_ = [("leaf_prims", C.c_int32), ("packet_budget", C.c_int32), ("traversal", C.c_int32),
-                ("wave_front", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("wave_front", C.c_int32), ("sil_pairs", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Records(C.Structure):

@@ -48,9 +48,10 @@ constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its o
 // Packet steps per wavefront before its queries are deferred to the wave-per-query pass, by query
 // kind and dimension (measured optimum with 6-primitive leaves and the two-phase wave pass on 1M
 // uniform queries: 3D closest point 7.73 ms at 1000 vs 8.32 at 800 and 7.92 at 1200; 2D closest
-// point 2.44 ms at 600 vs 2.52 at 750; silhouettes flat within 1% over 400-1000).  A scene or job
-// option overrides it.
-constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil = 750;
+// point 2.44 ms at 600 vs 2.52 at 750; 3D silhouettes, with near-last appends in the wave pass,
+// 29.05 ms at 128 vs 29.07 at 64, 29.30 at 250 and 31.47 at 750: the wave pass is the better
+// place for nearly all of their work).  A scene or job option overrides it.
+constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil = 128;
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
 

@@ -48,6 +48,7 @@ struct WideView {
     int packet_budget;        // 0 = default, < 0 = no deferral to the wave-per-query pass
     int traversal;            // silhouettes: 0 auto, 1 packet + wave pass, 2 lane per query
     int wave_front;           // first wave-pass frontier entries (0 = default)
+    int sil_pairs;            // 3D silhouettes under the quantized method: 0 float32 pairs, 1 8-bit
 };
 
 struct QueryArgs {

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 LEAVES = (2, 3, 4, 6)
-BUDGETS = (500, 750, 1000, 1500, 2000)
+BUDGETS = (128, 500, 1000, 1500, 2000)
 TRAVERSALS = (1, 2)
 
 

@@ -38,7 +38,8 @@ extern "C" {
 /* Traversal method of a query call. */
 #define GQ_METHOD_PLAIN 0       /* reference layout: binary AoS float32 nodes, lane per query */
 #define GQ_METHOD_ENGINE_FP32 1 /* wide SoA nodes with float32 bounds, sorted queries */
-#define GQ_METHOD_ENGINE 2      /* wide nodes with 8-bit (3D) / 16-bit (2D) quantized bounds+cones */
+#define GQ_METHOD_ENGINE 2      /* wide nodes with 8-bit (3D) / 16-bit (2D) quantized bounds; 3D
+                                   silhouettes on float32 pairs unless sil_pairs = 1 (below) */
 
 typedef struct gq_structure gq_structure; /* host-side BVH or SNCH (CPU builder output) */
 typedef struct gq_scene gq_scene;         /* device-resident scene */
@@ -108,14 +109,17 @@ typedef struct gq_scene_options {
     int32_t leaf_prims;    /* max primitives per engine BVH4 leaf, 1..16; 0 = default (6) */
     int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
                               move to the wave-per-query pass; 0 = default (closest point 1000 in
-                              3D / 600 in 2D, silhouettes 750), < 0 = never */
+                              3D / 600 in 2D, silhouettes 128), < 0 = never */
     int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D, packet + wave
                               pass in 3D), 1 = packet + wave pass, 2 = lane per query */
     int32_t wave_front;    /* LDS frontier entries of the first wave-per-query pass, 1..8192;
                               0 = default (closest point 1024, silhouettes 640).  A query whose
                               frontier outgrows it is finished by a second pass with the full
                               frontier: results do not change */
-    int32_t reserved[4];
+    int32_t sil_pairs;     /* 3D silhouette pair records under GQ_METHOD_ENGINE: 0 = float32 (128 B
+                              per pair, default: tighter boxes, fewer nodes opened), 1 = 8-bit
+                              quantized (64 B per pair, half the memory).  Results do not change */
+    int32_t reserved[3];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
 /* info[0..7] = dim, has_snch, device bytes, n_nodes, n_wide_nodes, n_prims, n_silhouettes, 0 */
