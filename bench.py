@@ -396,8 +396,10 @@ def main():
                     help="first wave-per-query pass frontier entries (0 = default)")
     ap.add_argument("--packet-budget", type=int, default=0,
                     help="packet steps per wavefront before deferral (0 = default, -1 = never)")
-    ap.add_argument("--query", default="cp", choices=["cp", "ray", "sil"],
-                    help="query type to time (the headline metric is cp)")
+    ap.add_argument("--query", default="cp", choices=["cp", "ray", "sil", "sample"],
+                    help="query type to time (the headline metric is cp; sample = BVH-guided boundary "
+                         "sampling in a ball of --radius)")
+    ap.add_argument("--radius", type=float, default=0.5, help="--query sample: ball radius")
     ap.add_argument("--n", type=int, default=N_QUERIES, help="queries per step")
     ap.add_argument("--subdiv", type=int, default=SUBDIV)
     ap.add_argument("--shape", default="mesh", choices=["mesh", "polyline"],
@@ -487,8 +489,13 @@ def main():
 
     method = args.method
     stream = torch.cuda.current_stream()
+    if args.query == "sample":
+        from gquery_amd.sampler import PCG32
+        gen = PCG32(nq, device="cuda:0")
 
     def step():
+        if args.query == "sample":
+            return mesh.sample_boundary(x, args.radius, gen)
         if args.query == "cp":
             return mesh.closest_point_bvh(x, method=method)
         if args.query == "ray":
@@ -523,9 +530,23 @@ def main():
         f"traversal kernel {kern_name} {kern_ms / max(kern_launches, 1):.3f} ms/launch")
 
     if args.query != "cp" or args.subdiv != SUBDIV or nq != N_QUERIES or dim != 3:
-        print(json.dumps({"shape": args.shape, "query": args.query, "method": method,
-                          "prims": int(F.shape[0]), "n": nq,
-                          "ms_per_step": ms_per_step, "Mq_per_s": value, "kernel_ms": kern_ms / max(kern_launches, 1)}))
+        line = {"shape": args.shape, "query": args.query, "method": method, "prims": int(F.shape[0]), "n": nq,
+                "ms_per_step": ms_per_step, "Mq_per_s": value, "kernel_ms": kern_ms / max(kern_launches, 1)}
+        if args.query == "sample":
+            line["radius"] = args.radius
+            line["hit_fraction"] = float(rec[0].float().mean())
+            if not args.no_cpu:
+                # the CPU restatement of the walk on the same queries, all host threads
+                from oracle import oracle as O
+                st, inc = O.pcg32_seed(nq)
+                t1 = time.perf_counter()
+                O.sample_boundary(dim, mesh.snch.bvh_node_data(), mesh.snch.primitive_data(), x_host,
+                                  args.radius, st, inc)
+                cpu_s = time.perf_counter() - t1
+                line["cpu_baseline"] = {"value": nq / cpu_s / 1e6, "unit": "Msamples/s",
+                                        "cores": O.num_threads(), "kind": "port",
+                                        "sample": f"all {nq} queries, {cpu_s:.2f} s"}
+        print(json.dumps(line))
         return
 
     # roofline of the dominant (traversal) kernel

@@ -39,6 +39,10 @@ class Records(C.Structure):
     ]
 
 
+class SampleRecords(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in ("hit", "p", "n", "uv", "pdf", "pmf", "prim_id")]
+
+
 QUERY_CLOSEST_POINT = 0
 QUERY_INTERSECT = 1
 QUERY_SILHOUETTE = 2
@@ -87,6 +91,7 @@ SIGNATURES = {
     "gq_closest_point_bvh": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_intersect_bvh": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_closest_silhouette_snch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
+    "gq_sample_boundary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(SampleRecords), C.c_void_p]),
     "gq_structure_engine_layout": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, _i64p]),
     "gq_batch_sizes": (C.c_int, [C.c_int, C.POINTER(Job), _i64p, _i64p]),
     "gq_batch_enqueue": (C.c_int, [C.c_int, C.POINTER(Job), C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]),

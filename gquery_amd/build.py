@@ -28,6 +28,7 @@ HIP_FLAGS = [
 HIP_UNITS = {
     "plain.hip": ["-ffp-contract=off"],
     "baseline.hip": ["-ffp-contract=off"],
+    "sample.hip": ["-ffp-contract=off"],
     "engine.hip": ["-ffp-contract=off"],
     "gq_api.hip": [],
 }

@@ -391,6 +391,19 @@ int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, cons
     return run_query(scene, kSilhouette, method, q, stream);
 }
 
+int gq_sample_boundary(gq_scene *scene, const float *x, const float *R, float R_scalar, uint64_t *rng_state,
+                       const uint64_t *rng_inc, int64_t n, const gq_sample_records *out, void *stream) {
+    if (!scene || !out) return fail(GQ_ERR_INVALID, "null argument");
+    if (n < 0) return fail(GQ_ERR_INVALID, "negative query count");
+    if (n == 0) return GQ_OK;
+    if (n > (int64_t)INT32_MAX) return fail(GQ_ERR_INVALID, "more than 2^31-1 queries in one call");
+    if (!x || !rng_state || !rng_inc) return fail(GQ_ERR_INVALID, "null query or generator pointer");
+    SampleArgs q{x, R, R_scalar, rng_state, rng_inc, n, *out};
+    const hipError_t e = launch_sample(scene->plain, q, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return GQ_OK;
+}
+
 static int batch_jobs(int n_jobs, const gq_job *jobs, std::vector<EngineJob> &out) {
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) return fail(GQ_ERR_INVALID, "bad job list");
     out.clear();

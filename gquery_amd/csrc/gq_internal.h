@@ -62,6 +62,19 @@ struct QueryArgs {
 
 enum QueryKind { kClosestPoint = 0, kIntersect = 1, kSilhouette = 2 };
 
+// Boundary sampling: query points, ball radii, per-query PCG32 state (advanced in place) and
+// stream increment.
+struct SampleArgs {
+    const float *x;
+    const float *R;           // optional per-query
+    float R_scalar;
+    uint64_t *state;
+    const uint64_t *inc;
+    int64_t n;
+    gq_sample_records out;
+};
+hipError_t launch_sample(const PlainView &s, const SampleArgs &q, hipStream_t st);
+
 hipError_t launch_plain(QueryKind kind, const PlainView &s, const QueryArgs &q, hipStream_t st);
 
 // Device view of a polyline / mesh for the brute-force (baseline) queries and the normals.

@@ -248,3 +248,30 @@ class Scene(_DeviceQueries):
             _lib.check(_lib.lib().gq_closest_silhouette_snch(self._h, m, _ptr(x), _ptr(rt), rs, n, C.byref(r),
                                                              self._stream()), "closest_silhouette_snch")
         return self._sil_record(out)
+
+    def sample_boundary(self, x, R, sampler):
+        """BVH3D.sample_boundary (bvh3d.py:246-346) on the scene's reference BVH: per query one
+        BVH-guided walk to a leaf, an area-weighted primitive among those meeting the ball (x, R)
+        and a uniform point on it.  `sampler`: a gquery_amd.sampler.PCG32 with one lane per query
+        on the scene's device, advanced in place.  Returns (hit, BoundarySamplingRecord3D) in 3D,
+        (hit, BoundarySamplingRecord) in 2D; a query without a sample is all zero."""
+        with torch.cuda.device(self.device):
+            x = as_points(x, self.dim, self.device)
+            n = x.shape[0]
+            if len(sampler) != n:
+                raise ValueError(f"sampler has {len(sampler)} lanes for {n} queries")
+            if sampler.state.device != self.device or sampler.inc.device != self.device:
+                raise ValueError("sampler must live on the scene's device")
+            rt, rs = self._r_max(R, n)
+            out, _ = self._alloc(n, ["p", "n", "uv", "prim_id"])
+            out["hit"] = torch.empty(n, device=self.device, dtype=torch.bool)
+            out["pdf"] = torch.empty(n, device=self.device, dtype=torch.float32)
+            out["pmf"] = torch.empty(n, device=self.device, dtype=torch.float32)
+            r = _lib.SampleRecords()
+            for name in ("hit", "p", "n", "uv", "pdf", "pmf", "prim_id"):
+                setattr(r, name, out[name].data_ptr())
+            _lib.check(_lib.lib().gq_sample_boundary(self._h, _ptr(x), _ptr(rt), rs, _ptr(sampler.state),
+                                                     _ptr(sampler.inc), n, C.byref(r), self._stream()),
+                       "sample_boundary")
+        cls = rec.BoundarySamplingRecord3D if self.dim == 3 else rec.BoundarySamplingRecord
+        return out["hit"], cls(out["p"], out["n"], out["uv"], out["pdf"], out["prim_id"], out["pmf"])

@@ -83,6 +83,12 @@ class Mesh:
         """SNCH3D.closest_silhouette (snch3d.py:143-203) -> ClosestSilhouettePointRecord3D."""
         return self.scene.closest_silhouette(p, r_max, method)
 
+    def sample_boundary(self, x, R, sampler):
+        """BVH3D.sample_boundary (bvh3d.py:246-346) over the mesh's BVH -> (hit,
+        BoundarySamplingRecord3D).  `sampler`: gquery_amd.sampler.PCG32 with one lane per query,
+        advanced in place."""
+        return self.scene.sample_boundary(x, R, sampler)
+
     def set_options(self, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0, wave_front: int = 0,
                     sil_pairs: int = 0):
         """Rebuilds the device scene with other engine options (the autotuner's choice)."""

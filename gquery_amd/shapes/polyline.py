@@ -70,6 +70,12 @@ class Polyline:
         """SNCH.closest_silhouette (snch.py:106-164) -> ClosestSilhouettePointRecord."""
         return self.scene.closest_silhouette(x, r_max, method)
 
+    def sample_boundary(self, x, R, sampler):
+        """The 2D form of BVH3D.sample_boundary (bvh3d.py:246-346) over the polyline's BVH, with
+        LineSegment.sample_point (line_segment.py:27-32) -> (hit, BoundarySamplingRecord).
+        `sampler`: gquery_amd.sampler.PCG32 with one lane per query, advanced in place."""
+        return self.scene.sample_boundary(x, R, sampler)
+
     def set_options(self, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0, wave_front: int = 0,
                     sil_pairs: int = 0):
         """Rebuilds the device scene with other engine options (the autotuner's choice)."""

@@ -54,6 +54,26 @@ class ClosestPointRecord3D:
 
 
 @dataclass
+class BoundarySamplingRecord:
+    p: torch.Tensor
+    n: torch.Tensor
+    t: torch.Tensor
+    pdf: torch.Tensor
+    prim_id: torch.Tensor
+    pmf: torch.Tensor
+
+
+@dataclass
+class BoundarySamplingRecord3D:
+    p: torch.Tensor
+    n: torch.Tensor
+    uv: torch.Tensor
+    pdf: torch.Tensor
+    prim_id: torch.Tensor
+    pmf: torch.Tensor
+
+
+@dataclass
 class ClosestSilhouettePointRecord:
     valid: torch.Tensor
     p: torch.Tensor

@@ -57,6 +57,19 @@ typedef struct gq_records {
     uint8_t *on_boundary; /* n         (intersection only)                             */
 } gq_records;
 
+/* Boundary sample records: BoundarySamplingRecord / BoundarySamplingRecord3D
+ * (gquery/shapes/primitive.py:51-67) and the hit flag BVH3D.sample_boundary returns beside them
+ * (bvh3d.py:247).  A query without a sample is all zero. */
+typedef struct gq_sample_records {
+    uint8_t *hit;     /* n                                                                  */
+    float *p;         /* n x dim   sampled point                                            */
+    float *n;         /* n x dim   unit normal of the sampled primitive                     */
+    float *uv;        /* 3D: n x 2 (u, v) weights of a and b; 2D: n  t along the segment   */
+    float *pdf;       /* n         pmf x density of the point on the primitive             */
+    float *pmf;       /* n         probability of the sampled primitive                    */
+    int32_t *prim_id; /* n         original primitive id                                   */
+} gq_sample_records;
+
 int gq_version(void);
 const char *gq_last_error(void);
 
@@ -145,6 +158,21 @@ int gq_intersect_bvh(gq_scene *scene, int method, const float *x, const float *v
  * Fills valid, p, d, prim_id. */
 int gq_closest_silhouette_snch(gq_scene *scene, int method, const float *x, const float *r_max,
                                float r_max_scalar, int64_t n, const gq_records *out, void *stream);
+
+/* BVH3D.sample_boundary (bvh3d.py:246-266) -> BVH3D._sample_boundary (bvh3d.py:268-346): per
+ * query one BVH-guided walk from the root to a leaf through children whose boxes meet the ball
+ * (x, R), an area-weighted choice among the leaf's primitives that meet the ball, and a uniform
+ * point on the chosen one (Triangle.sample_point, triangle.py:27-42).  Runs on the reference
+ * binary BVH of the scene.  R: per-query device array, or NULL to use R_scalar.  rng_state /
+ * rng_inc: per-query PCG32 state and odd stream increment (the two fields of DrJit's PCG32, the
+ * reference's `sampler`), device arrays; rng_state is advanced in place by the numbers drawn.
+ * Polylines run the same walk over segments (LineSegment.surface_area / sphere_intersect /
+ * sample_point, line_segment.py:23-36; the reference has these pieces but no 2D walk).  Where
+ * the reference cannot run as written (bvh3d.py:265,303 read fields that do not exist), the chosen
+ * primitive is its leaf-order slot and the record has no `type`; a leaf whose visible primitives
+ * all have zero area gives no sample. */
+int gq_sample_boundary(gq_scene *scene, const float *x, const float *R, float R_scalar, uint64_t *rng_state,
+                       const uint64_t *rng_inc, int64_t n, const gq_sample_records *out, void *stream);
 
 /* ---- Batches (no counterpart in the reference, which runs one DrJit-traced query per call; this
  * is the batched path for many scenes and query batches in one enqueue) ---- */

@@ -23,6 +23,10 @@
  * Part 2 (FP64): brute force over every primitive / silhouette, independent of any tree: nearest
  * primitive and runner-up distance (closest point), first and second hit (rays), nearest valid
  * silhouette and runner-up (silhouettes).  These pin the FP32 results to the mathematics.
+ *
+ * Part 3 (FP32): the brute-force *_baseline queries and mesh normals (gquery/shapes/polyline.py,
+ * mesh.py).  Part 4: BVH-guided boundary sampling (gquery/shapes/bvh3d.py:246-346) and the PCG32
+ * generator it draws from.
  */
 #include <math.h>
 #include <stdint.h>
@@ -892,4 +896,191 @@ int or_num_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ---- Part 4: BVH-guided boundary sampling ---------------------------------------------------
+ *
+ * or_sample_boundary follows BVH3D.sample_boundary / _sample_boundary (gquery/shapes/bvh3d.py:
+ * 246-346): one root-to-leaf walk per query.  At an interior node each child whose box overlaps
+ * the ball (x, R) gets weight 1 (branch_traversal_weight is 1, bvh3d.py:240-243); one uniform
+ * number picks a child with probability weight / total, which multiplies the pmf, and r_max
+ * becomes the chosen child's farthest-corner distance.  At the leaf, every primitive that is
+ * "visible" (r_max <= R: the whole leaf box lies in the ball, or the primitive meets the ball,
+ * Triangle.sphere_intersect triangle.py:44-47) joins a one-pass weighted reservoir with weight =
+ * surface area (triangle.py:23-25), drawing one number per visible primitive; the pmf gains
+ * area / total.  Then Triangle.sample_point (triangle.py:27-42) draws two numbers:
+ * u1 = sqrt(r0), u = 1 - u1, v = r1 * u1, w = 1 - u - v, p = a*u + b*v + c*w, pdf = 2 / |cross|.
+ * Record: p, n (unit face normal), uv = (u, v), pdf = pmf * pdf_point, pmf, prim_id = original
+ * index; a query whose root box misses the ball, or with no visible primitive, gets hit = 0 and
+ * an all-zero record.
+ *
+ * Where the reference cannot run as written, the meaning it states is taken:
+ *   - `prim.sorted_index` (bvh3d.py:303; no such field on Triangle) is the leaf-order slot of the
+ *     primitive, reference_offset + j, which is what self.primitives[...] indexes;
+ *   - `type=prim.type` (bvh3d.py:265) is dropped: BoundarySamplingRecord3D has no such field;
+ *   - a leaf whose visible primitives all have zero area yields no sample (the reference would
+ *     gather primitive -1).
+ * 2D (stride-7 nodes, stride-5 segments) is the same walk with LineSegment.surface_area =
+ * |b - a|, sphere_intersect by closest_point_line_segment and LineSegment.sample_point
+ * (line_segment.py:23-36: t = r0, p = a + (b - a) t, pdf = 1 / |b - a|, n = normal()); the
+ * reference has these pieces and BoundarySamplingRecord (primitive.py:51-58) but no 2D walk.
+ *
+ * Random numbers: DrJit's PCG32 (drjit.auto.ad.PCG32, gquery/core/fwd.py:5; DrJit is neither
+ * pinned by the reference nor installed here).  Its generator is O'Neill's pcg32 (XSH RR 64/32):
+ * state' = state * 6364136223846793005 + inc, output = rotr32(((old >> 18) ^ old) >> 27,
+ * old >> 59); next_float32 = float bits ((out >> 9) | 0x3f800000) - 1; seed(initstate, initseq):
+ * inc = (initseq << 1) | 1, state = 0, step, state += initstate, step.  The per-lane stream of a
+ * PCG32 of size n is initseq + lane.  Each query carries its own (state, inc); the state is
+ * advanced in place, as the DrJit sampler object is.
+ */
+static inline uint32_t pcg32_next(uint64_t *state, uint64_t inc) {
+    const uint64_t old = *state;
+    *state = old * 6364136223846793005ULL + inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+
+static inline float pcg32_float(uint64_t *state, uint64_t inc) {
+    const uint32_t b = (pcg32_next(state, inc) >> 9) | 0x3f800000u;
+    float f;
+    memcpy(&f, &b, 4);
+    return f - 1.0f;
+}
+
+void or_pcg32_seed(int64_t n, uint64_t initstate, uint64_t initseq, uint64_t *state, uint64_t *inc) {
+    for (int64_t i = 0; i < n; ++i) {
+        inc[i] = ((initseq + (uint64_t)i) << 1u) | 1u;
+        state[i] = 0;
+        pcg32_next(&state[i], inc[i]);
+        state[i] += initstate;
+        pcg32_next(&state[i], inc[i]);
+    }
+}
+
+/* k outputs of next_uint32 per lane, row-major (lane, k); states advanced in place */
+void or_pcg32_stream(uint64_t *state, const uint64_t *inc, int64_t n, int64_t k, uint32_t *out) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < k; ++j) out[i * k + j] = pcg32_next(&state[i], inc[i]);
+}
+
+typedef struct { void *hit, *p, *n, *uv, *pdf, *pmf, *prim_id; } or_sample_out;
+
+static inline float prim_area(int D, const float *pr) {
+    if (D == 3) {
+        float ab[3], ac[3], c[3];
+        for (int k = 0; k < 3; ++k) { ab[k] = pr[3 + k] - pr[k]; ac[k] = pr[6 + k] - pr[k]; }
+        cross3f(ab, ac, c);
+        return normf(3, c) / 2.0f;
+    }
+    float ba[2] = {pr[2] - pr[0], pr[3] - pr[1]};
+    return normf(2, ba);
+}
+
+static inline int prim_in_ball(int D, const float *pr, const float *x, float R) {
+    float pt[3], uv[2];
+    const float d = D == 3 ? cp_triangle(x, pr, pr + 3, pr + 6, pt, uv) : cp_segment(2, x, pr, pr + 2, pt);
+    return d <= R;
+}
+
+void or_sample_boundary(int D, const float *nodes, const float *prims, const float *x, const float *R_arr,
+                        float R_scalar, uint64_t *state, const uint64_t *inc, int64_t n, or_sample_out o) {
+    const int NS = D == 3 ? 9 : 7, PS = D == 3 ? 10 : 5;
+    uint8_t *hit_o = (uint8_t *)o.hit;
+    float *p_o = (float *)o.p, *n_o = (float *)o.n, *uv_o = (float *)o.uv, *pdf_o = (float *)o.pdf,
+          *pmf_o = (float *)o.pmf;
+    int32_t *id_o = (int32_t *)o.prim_id;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        const float *xi = x + D * i;
+        const float R = R_arr ? R_arr[i] : R_scalar;
+        uint64_t st = state[i];
+        const uint64_t in = inc[i];
+        int hit = 0, slot = -1;
+        float pdf = 1.0f, dmin, dmax;
+        box_dist(D, nodes, nodes + D, xi, &dmin, &dmax);
+        if (dmin <= R) {
+            float r_max = INFINITY;
+            int64_t node = 0;
+            for (;;) {
+                const float *nd = nodes + (int64_t)NS * node;
+                const int n_refs = (int)nd[2 * D + 2];
+                if (n_refs > 0) {
+                    const int off = (int)nd[2 * D];
+                    float total = 0.0f;
+                    for (int j = 0; j < n_refs; ++j) {
+                        const float *pr = prims + (int64_t)PS * (off + j);
+                        const float area = prim_area(D, pr);
+                        if ((r_max <= R) || prim_in_ball(D, pr, xi, R)) {
+                            hit = 1;
+                            total += area;
+                            const float sel = area / total;
+                            const float u = pcg32_float(&st, in);
+                            if (u < sel) slot = off + j;
+                        }
+                    }
+                    if (total > 0.0f) pdf *= prim_area(D, prims + (int64_t)PS * slot) / total;
+                    break;
+                }
+                const int64_t left = node + 1, right = node + (int64_t)(int)nd[2 * D + 1];
+                float dmin0, dmax0, dmin1, dmax1;
+                const float *b0 = nodes + (int64_t)NS * left, *b1 = nodes + (int64_t)NS * right;
+                box_dist(D, b0, b0 + D, xi, &dmin0, &dmax0);
+                box_dist(D, b1, b1 + D, xi, &dmin1, &dmax1);
+                const float w0 = dmin0 <= R ? 1.0f : 0.0f, w1 = dmin1 <= R ? 1.0f : 0.0f;
+                const float tw = w0 + w1;
+                if (!(tw > 0.0f)) break;
+                const float p0 = w0 / tw, p1 = 1.0f - p0;
+                const float u = pcg32_float(&st, in);
+                if (u < p0) { node = left; pdf *= p0; r_max = dmax0; }
+                else { node = right; pdf *= p1; r_max = dmax1; }
+            }
+        }
+        if (slot < 0) hit = 0;
+        if (hit_o) hit_o[i] = (uint8_t)hit;
+        float P[3] = {0, 0, 0}, N[3] = {0, 0, 0}, UV[2] = {0, 0}, pdf_rec = 0.0f, pmf_rec = 0.0f;
+        int32_t id = 0;
+        if (hit) {
+            const float *pr = prims + (int64_t)PS * slot;
+            if (D == 3) {
+                const float u1 = sqrtf(pcg32_float(&st, in));
+                const float u2 = pcg32_float(&st, in);
+                const float u = 1.0f - u1, v = u2 * u1, w = 1.0f - u - v;
+                float ab[3], ac[3], c[3];
+                for (int k = 0; k < 3; ++k) {
+                    P[k] = pr[k] * u + pr[3 + k] * v + pr[6 + k] * w;
+                    ab[k] = pr[3 + k] - pr[k];
+                    ac[k] = pr[6 + k] - pr[k];
+                }
+                cross3f(ab, ac, c);
+                const float pdf_pt = 2.0f / normf(3, c);
+                memcpy(N, c, 12);
+                normalizef(3, N);
+                UV[0] = u;
+                UV[1] = v;
+                pdf_rec = pdf * pdf_pt;
+                id = (int32_t)pr[9];
+            } else {
+                const float t = pcg32_float(&st, in);
+                const float ba[2] = {pr[2] - pr[0], pr[3] - pr[1]};
+                P[0] = pr[0] + ba[0] * t;
+                P[1] = pr[1] + ba[1] * t;
+                const float pdf_pt = 1.0f / normf(2, ba);
+                N[0] = ba[1];
+                N[1] = -ba[0];
+                normalizef(2, N);
+                UV[0] = t;
+                pdf_rec = pdf * pdf_pt;
+                id = (int32_t)pr[4];
+            }
+            pmf_rec = pdf;
+        }
+        if (p_o) for (int k = 0; k < D; ++k) p_o[D * i + k] = P[k];
+        if (n_o) for (int k = 0; k < D; ++k) n_o[D * i + k] = N[k];
+        if (uv_o) for (int k = 0; k < D - 1; ++k) uv_o[(D - 1) * i + k] = UV[k];
+        if (pdf_o) pdf_o[i] = pdf_rec;
+        if (pmf_o) pmf_o[i] = pmf_rec;
+        if (id_o) id_o[i] = id;
+        state[i] = st;
+    }
 }

@@ -28,6 +28,10 @@ class _Out(C.Structure):
                 ["valid", "p", "n", "uv", "d", "prim_id", "on_boundary", "visits", "child_tests", "leaf_tests"]]
 
 
+class _SampleOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ["hit", "p", "n", "uv", "pdf", "pmf", "prim_id"]]
+
+
 _lib = None
 
 
@@ -51,6 +55,10 @@ def lib():
         h.or_cp_baseline.argtypes = [C.c_int, vp, vp, i64, vp, vp, i64, _Out]
         h.or_ray_baseline.argtypes = [C.c_int, vp, vp, i64, vp, vp, vp, vp, vp, f32, i64, _Out]
         h.or_sil_baseline.argtypes = [C.c_int, vp, i64, vp, vp, f32, i64, _Out]
+        u64 = C.c_uint64
+        h.or_pcg32_seed.argtypes = [i64, u64, u64, vp, vp]
+        h.or_pcg32_stream.argtypes = [vp, vp, i64, i64, vp]
+        h.or_sample_boundary.argtypes = [C.c_int, vp, vp, vp, vp, f32, vp, vp, i64, _SampleOut]
         _lib = h
     return _lib
 
@@ -226,3 +234,48 @@ def configure_silhouette(V, E):
             continue
         out.append([a[0], a[1], b[0], b[1], c[0], c[1], p, u, q, i])
     return np.asarray(out, np.float32).reshape(-1, 10)
+
+
+def pcg32_seed(n, initstate=0x853C49E6748FEA9B, initseq=0xDA3E39CB94B95BDB):
+    """(state, inc) uint64 arrays of a PCG32 of size n seeded as DrJit does (stream initseq + lane)."""
+    state = np.zeros(n, np.uint64)
+    inc = np.zeros(n, np.uint64)
+    lib().or_pcg32_seed(n, initstate & (2**64 - 1), initseq & (2**64 - 1), state.ctypes.data, inc.ctypes.data)
+    return state, inc
+
+
+def pcg32_stream(state, inc, k):
+    """k next_uint32 outputs per lane (n, k); advances `state` in place."""
+    state = np.ascontiguousarray(state, np.uint64)
+    inc = np.ascontiguousarray(inc, np.uint64)
+    out = np.zeros((len(state), k), np.uint32)
+    lib().or_pcg32_stream(state.ctypes.data, inc.ctypes.data, len(state), k, out.ctypes.data)
+    return out, state
+
+
+def sample_boundary(dim, node_data, primitive_data, x, R, state, inc):
+    """BVH-guided boundary sampling (bvh3d.py:246-346 and its 2D analogue) over the reference
+    BVH arrays.  R: scalar or per-query array.  Returns (records, new_state)."""
+    x = _c(x, np.float32).reshape(-1, dim)
+    n = x.shape[0]
+    nodes = _c(node_data, np.float32)
+    prims = _c(primitive_data, np.float32)
+    st = np.array(state, dtype=np.uint64, copy=True).reshape(-1)
+    inc = _c(inc, np.uint64).reshape(-1)
+    assert st.shape == (n,) and inc.shape == (n,)
+    Ra = None
+    Rs = 0.0
+    if np.ndim(R) == 0:
+        Rs = float(R)
+    else:
+        Ra = _c(R, np.float32).reshape(-1)
+        assert Ra.shape == (n,)
+    r = {"hit": np.zeros(n, np.uint8), "p": np.zeros((n, dim), np.float32), "n": np.zeros((n, dim), np.float32),
+         "uv": np.zeros((n, 2), np.float32) if dim == 3 else np.zeros(n, np.float32),
+         "pdf": np.zeros(n, np.float32), "pmf": np.zeros(n, np.float32), "prim_id": np.zeros(n, np.int32)}
+    o = _SampleOut()
+    for k, v in r.items():
+        setattr(o, k, v.ctypes.data)
+    lib().or_sample_boundary(dim, nodes.ctypes.data, prims.ctypes.data, x.ctypes.data,
+                             None if Ra is None else Ra.ctypes.data, Rs, st.ctypes.data, inc.ctypes.data, n, o)
+    return r, st

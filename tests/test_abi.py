@@ -33,15 +33,17 @@ def test_bindings_cover_the_header():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "gquery_amd.h"\nint main(void) {\n'
-                   '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(gq_records), sizeof(gq_scene_options), sizeof(gq_job),\n'
-                   '         offsetof(gq_job, n), offsetof(gq_job, out), offsetof(gq_job, traversal));\n  return 0;\n}\n')
+                   '  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(gq_records), sizeof(gq_scene_options), sizeof(gq_job),\n'
+                   '         offsetof(gq_job, n), offsetof(gq_job, out), offsetof(gq_job, traversal),\n'
+                   '         sizeof(gq_sample_records), offsetof(gq_sample_records, prim_id));\n  return 0;\n}\n')
     exe = tmp_path / "sizes"
     r = subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], capture_output=True, text=True)
     if r.returncode != 0:
         pytest.skip("no C compiler: " + r.stderr[:200])
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(_lib.Records), C.sizeof(_lib.SceneOptions), C.sizeof(_lib.Job),
-            _lib.Job.n.offset, _lib.Job.out.offset, _lib.Job.traversal.offset]
+            _lib.Job.n.offset, _lib.Job.out.offset, _lib.Job.traversal.offset,
+            C.sizeof(_lib.SampleRecords), _lib.SampleRecords.prim_id.offset]
     assert got == want
 
 

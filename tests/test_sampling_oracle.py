@@ -1,0 +1,260 @@
+"""CPU: the oracle of BVH-guided boundary sampling (bvh3d.py:246-346) and of its PCG32 generator,
+pinned to what the mathematics fixes rather than to a second copy of the walk:
+
+  * PCG32: the published known-answer vector of O'Neill's pcg32 (pcg32_srandom_r(42, 54), first six
+    outputs of pcg32_random_r, as printed by the pcg-c-basic demo) and the seeding convention of the
+    host-side helper;
+  * the probability the walk reports (pmf) equals the probability of reaching that primitive,
+    computed by enumerating every root-to-leaf branch in float64;
+  * sampled primitive frequencies follow those probabilities (chi-square);
+  * the estimator mean(hit / pdf) is unbiased for the total area (length) of the primitives that
+    meet the ball, computed by FP64 brute force over all primitives;
+  * points lie on the sampled primitive, normals are its unit normal, pdf = pmf / area, and the
+    points are uniform on a triangle (barycentric means 1/3).
+"""
+import numpy as np
+import pytest
+
+from gquery_amd import ext, sampler as S, synth
+from oracle import oracle as O
+
+KNOWN_PCG32_42_54 = [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+
+
+def test_pcg32_known_answer():
+    st, inc = O.pcg32_seed(1, 42, 54)
+    out, _ = O.pcg32_stream(st, inc, 6)
+    assert [int(v) for v in out[0]] == KNOWN_PCG32_42_54
+    g = S.PCG32(1, 42, 54, device="cpu")
+    assert [int(g.next_uint32()[0]) for _ in range(6)] == KNOWN_PCG32_42_54
+
+
+def test_pcg32_seeding_and_floats_match_host_helper():
+    rng = np.random.default_rng(0)
+    for _ in range(4):
+        a, b = (int(v) for v in rng.integers(0, 2**63, size=2, dtype=np.int64))
+        s1, i1 = S.seed_arrays(1000, a, b)
+        s2, i2 = O.pcg32_seed(1000, a, b)
+        assert np.array_equal(s1, s2) and np.array_equal(i1, i2)
+        assert (i1 & np.uint64(1)).all()                   # odd increments
+    g = S.PCG32(257, device="cpu")
+    st, inc = S.seed_arrays(257)
+    out, _ = O.pcg32_stream(st, inc, 3)
+    for k in range(3):
+        f = g.next_float32().numpy()
+        want = ((out[:, k] >> np.uint32(9)) | np.uint32(0x3F800000)).view(np.float32) - np.float32(1)
+        assert np.array_equal(f, want) and (f >= 0).all() and (f < 1).all()
+
+
+def _geometry(dim):
+    if dim == 3:
+        V, F = synth.displaced_icosphere(2)            # 320 triangles
+        return V, F, ext.SNCH3D(V, F)
+    V, F = synth.star_polyline(256)
+    return V, F, ext.SNCH(V, F)
+
+
+def _prim_dist64(dim, P, x):
+    """FP64 distance from x to each primitive row of P (triangles a|b|c or segments a|b)."""
+    x = np.asarray(x, np.float64)
+    if dim == 2:
+        a, b = P[:, 0:2], P[:, 2:4]
+        d = b - a
+        t = np.clip(((x - a) * d).sum(1) / (d * d).sum(1), 0, 1)
+        return np.linalg.norm(a + t[:, None] * d - x, axis=1)
+    a, b, c = P[:, 0:3], P[:, 3:6], P[:, 6:9]
+
+    def seg(u, v):
+        d = v - u
+        t = np.clip(((x - u) * d).sum(1) / np.maximum((d * d).sum(1), 1e-300), 0.0, 1.0)
+        return np.linalg.norm(u + t[:, None] * d - x, axis=1)
+
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    h = ((x - a) * n).sum(1)
+    q = x - h[:, None] * n
+    inside = ((np.cross(b - a, q - a) * n).sum(1) >= 0) & ((np.cross(c - b, q - b) * n).sum(1) >= 0) & (
+        (np.cross(a - c, q - c) * n).sum(1) >= 0)
+    return np.where(inside, np.abs(h), np.minimum(np.minimum(seg(a, b), seg(b, c)), seg(c, a)))
+
+
+def _areas64(dim, P):
+    if dim == 2:
+        return np.linalg.norm(P[:, 2:4] - P[:, 0:2], axis=1)
+    return 0.5 * np.linalg.norm(np.cross(P[:, 3:6] - P[:, 0:3], P[:, 6:9] - P[:, 0:3]), axis=1)
+
+
+def _exact_probabilities(dim, nodes, prims, x, R):
+    """{original primitive id: probability the walk returns it}, by enumerating every branch the
+    walk can take (float64)."""
+    NS, PS = 2 * dim + 3, (10 if dim == 3 else 5)
+    nodes = np.asarray(nodes, np.float64).reshape(-1, NS)
+    prims = np.asarray(prims, np.float64).reshape(-1, PS)
+    x = np.asarray(x, np.float64)
+
+    def box(i):
+        lo, hi = nodes[i, :dim], nodes[i, dim:2 * dim]
+        u, v = lo - x, x - hi
+        return np.linalg.norm(np.maximum(np.maximum(u, v), 0)), np.linalg.norm(np.minimum(u, v))
+
+    out = {}
+    if box(0)[0] > R:
+        return out
+    stack = [(0, 1.0, np.inf)]
+    while stack:
+        node, p, r_max = stack.pop()
+        n_refs = int(nodes[node, 2 * dim + 2])
+        if n_refs > 0:
+            off = int(nodes[node, 2 * dim])
+            P = prims[off:off + n_refs]
+            vis = np.full(n_refs, True) if r_max <= R else _prim_dist64(dim, P, x) <= R
+            area = _areas64(dim, P)
+            tot = area[vis].sum()
+            if tot > 0:
+                for j in np.where(vis)[0]:
+                    pid = int(P[j, PS - 1])
+                    out[pid] = out.get(pid, 0.0) + p * area[j] / tot
+            continue
+        left, right = node + 1, node + int(nodes[node, 2 * dim + 1])
+        (d0, m0), (d1, m1) = box(left), box(right)
+        w0, w1 = float(d0 <= R), float(d1 <= R)
+        if w0 + w1 == 0:
+            continue
+        if w0:
+            stack.append((left, p * w0 / (w0 + w1), m0))
+        if w1:
+            stack.append((right, p * w1 / (w0 + w1), m1))
+    return out
+
+
+def _sample(dim, s, x, R, n, seed=7):
+    st, inc = O.pcg32_seed(n, 0x853C49E6748FEA9B + seed, 0xDA3E39CB94B95BDB)
+    X = np.repeat(np.asarray(x, np.float32)[None, :], n, 0)
+    r, st2 = O.sample_boundary(dim, s.bvh_node_data(), s.primitive_data(), X, R, st, inc)
+    return r, st, st2
+
+
+CASES = {3: [((0.9, 0.1, -0.2), 0.35), ((0.0, 0.0, 0.0), 1.2), ((1.3, 0.4, 0.2), 0.6)],
+         2: [((0.9, 0.1), 0.35), ((0.0, 0.0), 1.1), ((1.2, -0.3), 0.5)]}
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_pmf_is_the_exact_branch_probability(dim):
+    V, F, s = _geometry(dim)
+    for x, R in CASES[dim]:
+        exact = _exact_probabilities(dim, s.bvh_node_data(), s.primitive_data(), x, R)
+        assert exact, (x, R)
+        r, st, st2 = _sample(dim, s, x, R, 4000)
+        hit = r["hit"].astype(bool)
+        assert hit.any()
+        want = np.array([exact[int(i)] for i in r["prim_id"][hit]])
+        assert np.abs(r["pmf"][hit] / want - 1).max() < 2e-6
+        # the chance of a sample at all is the sum of the primitive probabilities
+        p_hit = sum(exact.values())
+        assert abs(hit.mean() - p_hit) < 5 * np.sqrt(p_hit * (1 - p_hit) / len(hit)) + 1e-9
+        assert not (r["pmf"][~hit]).any() and not (r["p"][~hit]).any()
+        assert (st2 != st).all()                           # every lane drew numbers
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_frequencies_follow_the_pmf(dim):
+    V, F, s = _geometry(dim)
+    x, R = CASES[dim][0]
+    exact = _exact_probabilities(dim, s.bvh_node_data(), s.primitive_data(), x, R)
+    n = 200_000
+    r, _, _ = _sample(dim, s, x, R, n, seed=11)
+    ids = np.array(sorted(exact))
+    expected = np.array([exact[i] for i in ids]) * n
+    counts = np.array([(r["prim_id"][r["hit"].astype(bool)] == i).sum() for i in ids])
+    big = expected >= 20
+    chi2 = (((counts[big] - expected[big]) ** 2) / expected[big]).sum()
+    dof = max(int(big.sum()) - 1, 1)
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+    assert abs(counts[~big].sum() - expected[~big].sum()) < 6 * np.sqrt(expected[~big].sum() + 1)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_estimator_is_unbiased_for_the_area_in_the_ball(dim):
+    V, F, s = _geometry(dim)
+    prims = np.asarray(s.primitive_data(), np.float64).reshape(-1, 10 if dim == 3 else 5)
+    for x, R in CASES[dim]:
+        truth = _areas64(dim, prims)[_prim_dist64(dim, prims, x) <= R].sum()
+        r, _, _ = _sample(dim, s, x, R, 200_000, seed=3)
+        hit = r["hit"].astype(bool)
+        est = np.where(hit, 1.0 / np.where(hit, r["pdf"].astype(np.float64), 1.0), 0.0)
+        se = est.std() / np.sqrt(len(est))
+        assert abs(est.mean() - truth) < 5 * se + 1e-6 * truth, (est.mean(), truth, se)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_points_normals_and_densities(dim):
+    V, F, s = _geometry(dim)
+    x, R = CASES[dim][1]
+    r, _, _ = _sample(dim, s, x, R, 20_000, seed=5)
+    hit = r["hit"].astype(bool)
+    ids = r["prim_id"][hit]
+    V64 = np.asarray(V, np.float64)
+    T = V64[np.asarray(F)[ids]]                             # (m, dim, dim) vertices per primitive
+    P = T.reshape(len(ids), -1)
+    p = r["p"][hit].astype(np.float64)
+    on = np.array([_prim_dist64(dim, P[k:k + 1], p[k])[0] for k in range(0, len(ids), 7)])
+    assert on.max() < 1e-6
+    area = _areas64(dim, P)
+    assert np.abs(r["pdf"][hit] * area / r["pmf"][hit] - 1).max() < 1e-5
+    if dim == 3:
+        nn = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+        nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+        uv = r["uv"][hit]
+        assert (uv >= 0).all() and (uv.sum(1) <= 1 + 1e-6).all()
+        q = T[:, 0] * uv[:, :1] + T[:, 1] * uv[:, 1:] + T[:, 2] * (1 - uv.sum(1))[:, None]
+        assert np.abs(q - p).max() < 1e-5
+    else:
+        d = T[:, 1] - T[:, 0]
+        nn = np.stack([d[:, 1], -d[:, 0]], 1) / np.linalg.norm(d, axis=1, keepdims=True)
+        t = r["uv"][hit]
+        assert (t >= 0).all() and (t < 1).all()
+        assert np.abs(T[:, 0] + t[:, None] * d - p).max() < 1e-5
+    assert np.abs(r["n"][hit] - nn).max() < 1e-6
+
+
+def test_uniform_points_on_one_triangle():
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 2, 0]], np.float32)
+    F = np.array([[0, 1, 2]], np.int32)
+    s = ext.SNCH3D(V, F)
+    n = 200_000
+    r, _, _ = _sample(3, s, (0.2, 0.2, 0.5), np.inf, n, seed=9)
+    assert r["hit"].all() and (r["pmf"] == 1).all() and np.allclose(r["pdf"], 1.0)   # area 1
+    uv = r["uv"].astype(np.float64)
+    w = 1 - uv.sum(1)
+    for m in (uv[:, 0], uv[:, 1], w):
+        assert abs(m.mean() - 1 / 3) < 5 * m.std() / np.sqrt(n)
+    assert np.abs(r["p"].mean(0) - V.mean(0)).max() < 5 * r["p"].std(0).max() / np.sqrt(n)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_edge_cases(dim):
+    V, F, s = _geometry(dim)
+    x0 = np.zeros(dim, np.float32)
+    x0[0] = 5.0
+    # a ball that misses the root box: no sample, no numbers drawn
+    r, st, st2 = _sample(dim, s, x0, 1.0, 64)
+    assert not r["hit"].any() and not r["pdf"].any() and not r["p"].any() and np.array_equal(st, st2)
+    # an infinite ball: every primitive is reachable, probabilities sum to one, always a sample
+    exact = _exact_probabilities(dim, s.bvh_node_data(), s.primitive_data(), x0, np.inf)
+    assert len(exact) == len(F) and abs(sum(exact.values()) - 1) < 1e-9
+    r, _, _ = _sample(dim, s, x0, np.inf, 512)
+    assert r["hit"].all()
+    # per-query radii equal the scalar calls
+    n = 300
+    X = synth.uniform_points(n, dim, seed=4)
+    Rs = np.linspace(0.05, 1.0, n).astype(np.float32)
+    st, inc = O.pcg32_seed(n, 1, 2)
+    ra, sa = O.sample_boundary(dim, s.bvh_node_data(), s.primitive_data(), X, Rs, st, inc)
+    for k in range(0, n, 37):
+        rk, sk = O.sample_boundary(dim, s.bvh_node_data(), s.primitive_data(), X[k:k + 1], float(Rs[k]),
+                                   st[k:k + 1], inc[k:k + 1])
+        assert sk[0] == sa[k] and all(np.array_equal(rk[f][0], ra[f][k]) for f in ra)
+    # no queries
+    r0, s0 = O.sample_boundary(dim, s.bvh_node_data(), s.primitive_data(), np.zeros((0, dim), np.float32), 1.0,
+                               np.zeros(0, np.uint64), np.zeros(0, np.uint64))
+    assert len(r0["hit"]) == 0 and len(s0) == 0
