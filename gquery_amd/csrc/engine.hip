@@ -50,8 +50,10 @@ constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its o
 // uniform queries: 3D closest point 7.73 ms at 1000 vs 8.32 at 800 and 7.92 at 1200; 2D closest
 // point 2.44 ms at 600 vs 2.52 at 750; 3D silhouettes, with near-last appends in the wave pass,
 // 29.05 ms at 128 vs 29.07 at 64, 29.30 at 250 and 31.47 at 750: the wave pass is the better
-// place for nearly all of their work).  A scene or job option overrides it.
-constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil = 128;
+// place for nearly all of their work; 2D silhouettes in packet mode (65,536 segments) 3.28 ms at
+// 750 vs 3.85 at 2,000 and 8.93 at 128: a 2D query has too little work for a whole wavefront).
+// A scene or job option overrides it.
+constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil3 = 128, kPacketBudgetSil2 = 750;
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
 
@@ -165,20 +167,16 @@ __device__ __forceinline__ GP gview(const EngineParams &in) {
     return P;
 }
 
-// ---- lane-per-query stacks.  GQ_LANE_STACK_DECL (2D silhouettes): LDS, depth chosen per launch
-// from the tree depth, one column per lane of a [depth][blockDim] dynamic shared array.
-// GQ_PRIVATE_STACK_DECL (rays): a per-lane array in private (scratch) memory, bounded by
-// kRayStackMax, so LDS does not limit how many wavefronts a CU holds; only the top entries are
-// touched and they stay in the L1 cache.  3D rays, 1M against 327,680 triangles: 0.50 ms vs
-// 0.68 ms with the LDS stack (2 wavefronts per SIMD); 2D silhouettes were slower this way
-// (2.82 vs 2.73 ms).
-#define GQ_LANE_STACK_DECL                     \
-    extern __shared__ uint2 lane_stack[];      \
-    const int lane_col = threadIdx.x;          \
-    const int lane_stride = blockDim.x;        \
-    int sp = 0;
-#define GQ_PRIVATE_STACK_DECL                  \
-    uint2 lane_stack[kRayStackMax];            \
+// ---- lane-per-query stacks.  In LDS: depth chosen per launch from the tree depth, one column per
+// lane of a [depth][blockDim] dynamic shared array.  In private (scratch) memory: a per-lane array
+// of the largest bound, so LDS does not limit how many wavefronts a CU holds; only the top entries
+// are touched and they stay in the L1 cache.  Measured on 1M queries: 3D rays against 327,680
+// triangles 0.50 ms private vs 0.68 ms in LDS (2 wavefronts per SIMD); 3D silhouettes per lane
+// 44.3 vs 51.9 ms (20,480 triangles: 15.7 vs 16.2 ms); 2D silhouettes (65,536 segments) 2.82 ms
+// private vs 2.73 in LDS.  GQ_PRIVATE_STACK_DECL(N) declares a private stack of N entries;
+// GQ_LPUSH / GQ_LPOP work on either kind through lane_stack, lane_col and lane_stride.
+#define GQ_PRIVATE_STACK_DECL(N)               \
+    uint2 lane_stack[N];                       \
     const int lane_col = 0;                    \
     const int lane_stride = 1;                 \
     int sp = 0;
@@ -667,7 +665,7 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
 template <int D, bool Q>
 __device__ __forceinline__ void ray_lane(const GP &P, int base) {
     using V = typename VT<D>::T;
-    GQ_PRIVATE_STACK_DECL
+    GQ_PRIVATE_STACK_DECL(kRayStackMax)
     const int i = base + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t qi = query_index<D>(P, i);
@@ -1126,12 +1124,18 @@ __device__ __forceinline__ void sil_record(const GP &P, uint32_t qi, const typen
 
 // ---------------------------------------------------------------------------------------------
 // closest silhouette, lane per query: the same node tests as the packet pass, one query per lane
-// over the Morton order with a per-lane LDS stack (one entry per binary level at most).  Cheaper
-// than the packet pass when per-query work is small and culling is effective (2D silhouettes).
+// over the Morton order with a per-lane stack (one entry per binary level at most): in LDS in 2D,
+// in private memory in 3D (measured above).  Cheaper than the packet pass when per-query work is
+// small and culling is effective (2D silhouettes).
 template <int D, bool Q>
 __device__ __forceinline__ void sil_lane(const GP &P, int base) {
     using V = typename VT<D>::T;
-    GQ_LANE_STACK_DECL
+    extern __shared__ uint2 lane_stack_lds[];
+    uint2 lane_stack_priv[D == 3 ? kPacketDepth : 1];
+    uint2 *const lane_stack = D == 3 ? lane_stack_priv : lane_stack_lds;
+    const int lane_col = D == 3 ? 0 : (int)threadIdx.x;
+    const int lane_stride = D == 3 ? 1 : (int)blockDim.x;
+    int sp = 0;
     const int i = base + threadIdx.x;
     if (i >= P.n) return;
     const uint32_t qi = query_index<D>(P, i);
@@ -1791,7 +1795,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             const int budget = jobs[j].budget != 0 ? jobs[j].budget : w.packet_budget;
             if (fits && budget >= 0) {
                 P.budget = budget > 0 ? budget
-                         : G.kind == kSilhouette ? kPacketBudgetSil
+                         : G.kind == kSilhouette ? (G.dim == 3 ? kPacketBudgetSil3 : kPacketBudgetSil2)
                          : G.dim == 3 ? kPacketBudgetCp3 : kPacketBudgetCp2;
                 G.defer = true;
                 G.front = std::max(G.front, need);
@@ -1880,7 +1884,8 @@ void launch_group_t(const Group &G, const EngineParams *J, const uint2 *tiles, u
     if (G.kind == kIntersect) {
         hipLaunchKernelGGL((k_ray_lane<D, Q>), dim3(nt), dim3(kRayBlock), 0, st, J, gt, xcd_spread());
     } else if (G.kind == kSilhouette && G.lane) {
-        hipLaunchKernelGGL((k_sil_lane<D, Q>), dim3(nt), dim3(kRayBlock), (size_t)G.max_stack * kRayBlock * sizeof(uint2), st, J, gt, xcd_spread());
+        const size_t lds = D == 3 ? 0 : (size_t)G.max_stack * kRayBlock * sizeof(uint2);
+        hipLaunchKernelGGL((k_sil_lane<D, Q>), dim3(nt), dim3(kRayBlock), lds, st, J, gt, xcd_spread());
     } else if (G.kind == kSilhouette) {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 4);
         hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt, xcd_spread());

@@ -122,7 +122,7 @@ typedef struct gq_scene_options {
     int32_t leaf_prims;    /* max primitives per engine BVH4 leaf, 1..16; 0 = default (6) */
     int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
                               move to the wave-per-query pass; 0 = default (closest point 1000 in
-                              3D / 600 in 2D, silhouettes 128), < 0 = never */
+                              3D / 600 in 2D, silhouettes 128 in 3D / 750 in 2D), < 0 = never */
     int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D, packet + wave
                               pass in 3D), 1 = packet + wave pass, 2 = lane per query */
     int32_t wave_front;    /* LDS frontier entries of the first wave-per-query pass, 1..8192;
