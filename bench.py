@@ -400,6 +400,8 @@ def main():
                     help="query type to time (the headline metric is cp; sample = BVH-guided boundary "
                          "sampling in a ball of --radius)")
     ap.add_argument("--radius", type=float, default=0.5, help="--query sample: ball radius")
+    ap.add_argument("--sil-pairs", type=int, default=0, choices=[0, 1],
+                    help="3D silhouettes under method engine: 0 float32 pair records, 1 8-bit")
     ap.add_argument("--n", type=int, default=N_QUERIES, help="queries per step")
     ap.add_argument("--subdiv", type=int, default=SUBDIV)
     ap.add_argument("--shape", default="mesh", choices=["mesh", "polyline"],
@@ -479,7 +481,7 @@ def main():
     else:
         V, F = synth.displaced_icosphere(args.subdiv)
         mesh = Mesh(V, F, device="cuda:0", leaf_prims=args.leaf_prims, packet_budget=args.packet_budget,
-                    traversal=args.traversal, wave_front=args.wave_front)
+                    traversal=args.traversal, wave_front=args.wave_front, sil_pairs=args.sil_pairs)
     log(f"scene built in {time.perf_counter() - t0:.1f}s: {F.shape[0]} primitives")
     nq = args.n
     x_host = synth.uniform_points(nq, dim)
