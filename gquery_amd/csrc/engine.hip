@@ -553,16 +553,11 @@ constexpr int kFrontSmallCp = 1024, kFrontSmallSil = 640;
 // slower (7.57-7.69 ms vs 7.47 ms).
 constexpr float kSilNear = 1.25f;
 
-__device__ __forceinline__ int wave_exclusive_scan(int v, int &total) {
-    const int lane = threadIdx.x & 63;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    total = __shfl(x, 63, 64);
-    return x - v;
+// Number of set bits of a wave mask in the lanes below this one (mbcnt): with one ballot per
+// child slot, the append position of a lane's children is the sum over slots, and the round's
+// total the sum of the masks' popcounts -- no shuffles through LDS.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Appends one query (index, bound, best slot, job) to a hand-off list: every lane takes part in the
@@ -655,11 +650,14 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                 best_slot = cand_slot;
                 bound2 = cand * cand * 1.000002f;
             }
-            // append the opened children
-            int tot;
-            const int nh = (int)hit[0] + (int)hit[1] + (int)hit[2] + (int)hit[3];
-            int pos = size + wave_exclusive_scan(nh, tot);
-            tot = (int)uniform((uint32_t)tot);
+            // append the opened children, in lane order and child order within a lane
+            int tot = 0, pos = size;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const unsigned long long b = __ballot(hit[j]);
+                tot += __popcll(b);
+                pos += lanes_below(b);
+            }
             if (ovf != nullptr && size + tot > cap) {   // wave-uniform: hand the query on
                 overflow = true;
                 break;
@@ -1335,11 +1333,11 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
             const float thr = ovf != nullptr ? fmaf(uniformf(km), kSilNear, 1e-6f) : kInf;
             const bool n0 = h0 && __uint_as_float(ch0.y) <= thr, n1 = h1 && __uint_as_float(ch1.y) <= thr;
             const bool f0 = h0 && !n0, f1 = h1 && !n1;
-            int tot_far, tot_near;
-            int pf = size + wave_exclusive_scan((int)f0 + (int)f1, tot_far);
-            tot_far = (int)uniform((uint32_t)tot_far);
-            int pn = size + tot_far + wave_exclusive_scan((int)n0 + (int)n1, tot_near);
-            const int tot = tot_far + (int)uniform((uint32_t)tot_near);
+            const unsigned long long bf0 = __ballot(f0), bf1 = __ballot(f1), bn0 = __ballot(n0), bn1 = __ballot(n1);
+            const int tot_far = __popcll(bf0) + __popcll(bf1);
+            const int tot = tot_far + __popcll(bn0) + __popcll(bn1);
+            int pf = size + lanes_below(bf0) + lanes_below(bf1);
+            int pn = size + tot_far + lanes_below(bn0) + lanes_below(bn1);
             if (ovf != nullptr && size + tot > cap) {   // wave-uniform: hand the query on
                 overflow = true;
                 break;
