@@ -553,6 +553,45 @@ constexpr int kFrontSmallCp = 1024, kFrontSmallSil = 640;
 // slower (7.57-7.69 ms vs 7.47 ms).
 constexpr float kSilNear = 1.25f;
 
+// Wave-wide minima through DPP lane moves (register to register, no LDS): swap within pairs and
+// quads, mirror half-rows and rows (every lane of a 16-lane row then holds the row's minimum), then
+// broadcast lane 15 into rows 1 and 3 and lane 31 into rows 2 and 3; lane 63 ends with the
+// minimum of the wave, which readlane hands to every lane.  Lanes a row mask leaves out keep their
+// own value (`old` = the value itself).  ArgMin orders (distance, slot) lexicographically, so ties
+// go to the lower slot.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void argmin_step(float &c, int &s) {
+    const float oc = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(c), __float_as_int(c), CTRL, ROWS, 0xF, false));
+    const int os = __builtin_amdgcn_update_dpp(s, s, CTRL, ROWS, 0xF, false);
+    if (oc < c || (oc == c && os < s)) {
+        c = oc;
+        s = os;
+    }
+}
+__device__ __forceinline__ void wave_argmin(float &c, int &s) {
+    argmin_step<0xB1, 0xF>(c, s);    // quad_perm [1,0,3,2]
+    argmin_step<0x4E, 0xF>(c, s);    // quad_perm [2,3,0,1]
+    argmin_step<0x141, 0xF>(c, s);   // row_half_mirror
+    argmin_step<0x140, 0xF>(c, s);   // row_mirror
+    argmin_step<0x142, 0xA>(c, s);   // row_bcast:15 into rows 1, 3
+    argmin_step<0x143, 0xC>(c, s);   // row_bcast:31 into rows 2, 3
+    c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), 63));
+    s = __builtin_amdgcn_readlane(s, 63);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float min_step(float v) {
+    return fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROWS, 0xF, false)));
+}
+__device__ __forceinline__ float wave_min(float v) {
+    v = min_step<0xB1, 0xF>(v);
+    v = min_step<0x4E, 0xF>(v);
+    v = min_step<0x141, 0xF>(v);
+    v = min_step<0x140, 0xF>(v);
+    v = min_step<0x142, 0xA>(v);
+    v = min_step<0x143, 0xC>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Number of set bits of a wave mask in the lanes below this one (mbcnt): with one ballot per
 // child slot, the append position of a lane's children is the sum over slots, and the round's
 // total the sum of the masks' popcounts -- no shuffles through LDS.
@@ -634,17 +673,7 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                 }
             }
             // wave minimum of the leaf candidates (ties to the lower slot, so every lane agrees)
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const float oc = __shfl_xor(cand, o, 64);
-                const int os = __shfl_xor(cand_slot, o, 64);
-                if (oc < cand || (oc == cand && os < cand_slot)) {
-                    cand = oc;
-                    cand_slot = os;
-                }
-            }
-            cand = uniformf(cand);
-            cand_slot = (int)uniform((uint32_t)cand_slot);
+            wave_argmin(cand, cand_slot);
             if (cand < best || (cand == best && cand_slot < best_slot)) {
                 best = cand;
                 best_slot = cand_slot;
@@ -1307,17 +1336,7 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
                     }
                 }
             }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const float oc = __shfl_xor(cand, o, 64);
-                const int os = __shfl_xor(cand_slot, o, 64);
-                if (oc < cand || (oc == cand && os < cand_slot)) {
-                    cand = oc;
-                    cand_slot = os;
-                }
-            }
-            cand = uniformf(cand);
-            cand_slot = (int)uniform((uint32_t)cand_slot);
+            wave_argmin(cand, cand_slot);
             if (cand < r || (cand == r && best_slot >= 0 && cand_slot >= 0 && cand_slot < best_slot)) {
                 r = cand;
                 best_slot = cand_slot;
@@ -1328,9 +1347,8 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
             // bound does (it needs lane-order appends), so the full-frontier pass (no hand-off
             // list) appends everything in lane order (thr = inf: one group).
             float km = fminf(h0 ? __uint_as_float(ch0.y) : kInf, h1 ? __uint_as_float(ch1.y) : kInf);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) km = fminf(km, __shfl_xor(km, o, 64));
-            const float thr = ovf != nullptr ? fmaf(uniformf(km), kSilNear, 1e-6f) : kInf;
+            km = wave_min(km);
+            const float thr = ovf != nullptr ? fmaf(km, kSilNear, 1e-6f) : kInf;
             const bool n0 = h0 && __uint_as_float(ch0.y) <= thr, n1 = h1 && __uint_as_float(ch1.y) <= thr;
             const bool f0 = h0 && !n0, f1 = h1 && !n1;
             const unsigned long long bf0 = __ballot(f0), bf1 = __ballot(f1), bn0 = __ballot(n0), bn1 = __ballot(n1);
