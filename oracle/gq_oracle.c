@@ -932,6 +932,12 @@ int or_num_threads(void) {
  * inc = (initseq << 1) | 1, state = 0, step, state += initstate, step.  The per-lane stream of a
  * PCG32 of size n is initseq + lane.  Each query carries its own (state, inc); the state is
  * advanced in place, as the DrJit sampler object is.
+ *
+ * Pins (tests/test_sampling_oracle.py): the generator step against the published pcg32
+ * known-answer vector; the walk's pmf against float64 enumeration of every branch; frequencies
+ * (chi-square); unbiasedness of 1/pdf for the FP64 brute-force area; point placement and density.
+ * Parity unpinned: the per-lane seeding convention (stream initseq + lane) -- DrJit is absent and
+ * no reference test or fixture fixes it; callers passing their own states do not depend on it.
  */
 static inline uint32_t pcg32_next(uint64_t *state, uint64_t inc) {
     const uint64_t old = *state;
