@@ -29,10 +29,9 @@ def _step(state: np.ndarray, inc: np.ndarray) -> np.ndarray:
         return state * np.uint64(MULT) + inc
 
 
-def _output(old: np.ndarray) -> np.ndarray:
-    xs = (((old >> np.uint64(18)) ^ old) >> np.uint64(27)).astype(np.uint32)
-    rot = (old >> np.uint64(59)).astype(np.uint32)
-    return (xs >> rot) | (xs << ((np.uint32(32) - rot) & np.uint32(31)))
+def _srl(x: torch.Tensor, s: int) -> torch.Tensor:
+    """Logical right shift of the uint64 bits held in an int64 tensor."""
+    return (x >> s) & ((1 << (64 - s)) - 1)
 
 
 def seed_arrays(size: int, initstate: int = DEFAULT_STATE, initseq: int = DEFAULT_STREAM):
@@ -63,13 +62,15 @@ class PCG32:
         return int(self.state.numel())
 
     def next_uint32(self) -> torch.Tensor:
-        """One output per lane (int64 tensor holding the uint32 value); advances every lane."""
-        old = self.state.cpu().numpy().view(np.uint64)
-        out = _output(old)
-        self.state = torch.from_numpy(_step(old, self.inc.cpu().numpy().view(np.uint64)).view(np.int64)).to(self.device)
-        return torch.from_numpy(out.astype(np.int64)).to(self.device)
+        """One output per lane (int64 tensor holding the uint32 value), computed where the state
+        lives with wrapping int64 arithmetic; advances every lane."""
+        old = self.state
+        self.state = old * MULT + self.inc                      # mod 2^64: int64 arithmetic wraps
+        xs = _srl(_srl(old, 18) ^ old, 27) & 0xFFFFFFFF
+        rot = _srl(old, 59)
+        return ((xs >> rot) | (xs << ((32 - rot) & 31))) & 0xFFFFFFFF
 
     def next_float32(self) -> torch.Tensor:
         """One uniform float32 in [0, 1) per lane; advances every lane."""
-        bits = ((self.next_uint32().cpu().numpy().astype(np.uint32) >> np.uint32(9)) | np.uint32(0x3F800000))
-        return torch.from_numpy(bits.view(np.float32) - np.float32(1.0)).to(self.device)
+        bits = (self.next_uint32() >> 9) | 0x3F800000
+        return bits.to(torch.int32).view(torch.float32) - 1.0

@@ -81,3 +81,17 @@ def test_sample_boundary_edge_cases(gpu, name):
         shape.sample_boundary(x, 1.0, PCG32(99, device="cuda:0"))
     with pytest.raises(ValueError):
         shape.sample_boundary(x, 1.0, PCG32(100, device="cpu"))
+
+
+def test_generator_draws_on_the_device(gpu):
+    """PCG32.next_uint32 / next_float32 run where the state lives (wrapping int64 tensor
+    arithmetic) and continue the same streams the sampling kernel advances."""
+    g = PCG32(4096, initstate=99, initseq=7, device="cuda:0")
+    st, inc = _u64(g.state), _u64(g.inc)
+    want, _ = O.pcg32_stream(st, inc, 4)
+    for k in range(3):
+        got = g.next_uint32()
+        assert got.device.type == "cuda"
+        assert np.array_equal(got.cpu().numpy().astype(np.uint32), want[:, k])
+    f = g.next_float32().cpu().numpy()
+    assert np.array_equal(f, ((want[:, 3] >> np.uint32(9)) | np.uint32(0x3F800000)).view(np.float32) - np.float32(1))
