@@ -838,6 +838,8 @@ struct PairChild {
     int32_t ref_cone;
     float half, radius;
     float sin_h, cos_h;   // 2D: sin and cos of the reference half-angle
+    float sil_lo, sil_hi, sil_perp;   // 3D float32 records: extent of the silhouette edges below
+                                      // along / across the cone axis about the box centre
 };
 
 template <int D, bool Q>
@@ -864,6 +866,7 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
             c[j].sin_t = 1.0f;
             c[j].sin_h = r.f32(88 + 4 * j);
             c[j].cos_h = r.f32(96 + 4 * j);
+            c[j].sil_lo = c[j].sil_hi = c[j].sil_perp = 0.0f;
         }
     } else if constexpr (Q) {
         // SPairQ<3> (64 B): origin, exp | child boxes | cone_axis int16 @28 | cone_cs int16 @40 |
@@ -899,10 +902,11 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
             c[j].sin_t = cs1 * kS;
             c[j].child = r.u32(48 + 4 * j);
             c[j].ref_cone = (int32_t)r.u32(56 + 4 * j);
+            c[j].sil_lo = c[j].sil_hi = c[j].sil_perp = 0.0f;
         }
     } else {
         // SPairF<D> (128 B): lo[2][D] @0, hi[2][D] @8D, cones (axis3, cos, sin) @16D, child @16D+40,
-        // ref_cone @16D+48
+        // ref_cone @16D+48, sil_lo @16D+56, sil_hi @16D+64, sil_perp @16D+72
         Words<32> r;
         r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 128);
 #pragma unroll
@@ -919,6 +923,9 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
             c[j].sin_t = r.f32(co + 16);
             c[j].child = r.u32(16 * D + 40 + 4 * j);
             c[j].ref_cone = (int32_t)r.u32(16 * D + 48 + 4 * j);
+            c[j].sil_lo = r.f32(16 * D + 56 + 4 * j);
+            c[j].sil_hi = r.f32(16 * D + 64 + 4 * j);
+            c[j].sil_perp = r.f32(16 * D + 72 + 4 * j);
         }
     }
 }
@@ -926,7 +933,7 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
 // True when every silhouette predicate in the child is provably false for query x: the normals
 // lie within the cone and the dot product of every view direction into the box with every cone
 // normal has one sign (bounded away from zero by a margin covering float32 rounding).
-template <int D>
+template <int D, bool Q>
 __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) {
     if (c.cos_t > 1.5f) return true;     // no silhouette can ever pass in this subtree
     if (!(c.cos_t > 0.0f)) return false; // cone too wide to separate anything
@@ -937,14 +944,31 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
             w[k] = 0.5f * (c.lo[k] + c.hi[k]) - xc[k];
             h[k] = 0.5f * (c.hi[k] - c.lo[k]);
         }
-        const float R = sqrtf(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
         const float a = w[0] * c.axis[0] + w[1] * c.axis[1] + w[2] * c.axis[2];
         const float cx = w[1] * c.axis[2] - w[2] * c.axis[1];
         const float cy = w[2] * c.axis[0] - w[0] * c.axis[2];
         const float cz = w[0] * c.axis[1] - w[1] * c.axis[0];
         const float b = sqrtf(cx * cx + cy * cy + cz * cz);
-        const float m = R + 1e-5f * (fabsf(a) + b + R) + 1e-6f;
-        return (a * c.cos_t - b * c.sin_t > m) || (a * c.cos_t + b * c.sin_t < -m);
+        if constexpr (Q) {
+            // quantized records: a sphere of the box's half-diagonal R around the centre bounds
+            // every edge point p, so dot(p - x, n) >= a cos - b sin - R for every cone normal n
+            const float R = sqrtf(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+            const float m = R + 1e-5f * (fabsf(a) + b + R) + 1e-6f;
+            return (a * c.cos_t - b * c.sin_t > m) || (a * c.cos_t + b * c.sin_t < -m);
+        } else {
+            // float32 records: the edges' extent along the axis, [lo_u, hi_u], and across it, rho.
+            // With n = cos(phi) u + sin(phi) t (phi <= theta, t _|_ u), dot(p - x, n) lies in
+            // [cos (a + lo_u) - sin (b + rho), cos (a + hi_u) + sin (b + rho)] at phi = theta, and
+            // the sphere through the extents (radius sqrt(max(lo_u^2, hi_u^2) + rho^2)) gives the
+            // bound above; either proof culls
+            const float lo_u = c.sil_lo, hi_u = c.sil_hi, rho = c.sil_perp;
+            const float R = sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho);
+            const float m = 1e-5f * (fabsf(a) + b + R) + 1e-6f;
+            const bool sphere = (a * c.cos_t - b * c.sin_t > R + m) || (a * c.cos_t + b * c.sin_t < -(R + m));
+            const bool slab = ((a + lo_u) * c.cos_t - (b + rho) * c.sin_t > m) ||
+                              ((a + hi_u) * c.cos_t + (b + rho) * c.sin_t < -m);
+            return sphere || slab;
+        }
     } else {
         return false;
     }
@@ -1053,7 +1077,7 @@ __device__ __forceinline__ void sil_packet(const GP &P, int base) {
                     box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                     bool h = act && dm <= r;
                     if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
-                    if (h && exact_cull<D>(c[j], xc)) h = false;
+                    if (h && exact_cull<D, Q>(c[j], xc)) h = false;
                     hh[j] = h;
                     hm[j] = __ballot(h);
                     any[j] = hm[j] != 0ull;
@@ -1210,7 +1234,7 @@ __device__ __forceinline__ void sil_lane(const GP &P, int base) {
                     float dr;
                     hj = ref_cone_pass<D>(P, c[j], x, r, dr);
                 }
-                if (hj && exact_cull<D>(c[j], xc)) hj = false;
+                if (hj && exact_cull<D, Q>(c[j], xc)) hj = false;
                 h[j] = hj;
                 key[j] = dm;
             }
@@ -1301,7 +1325,7 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
                         box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                         bool h = dm <= r;
                         if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
-                        if (h && exact_cull<D>(c[j], xc)) h = false;
+                        if (h && exact_cull<D, Q>(c[j], xc)) h = false;
                         const uint2 ent = make_uint2(c[j].child, __float_as_uint(dm));
                         if (j == 0) { h0 = h; ch0 = ent; }
                         else { h1 = h; ch1 = ent; }

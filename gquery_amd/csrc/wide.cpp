@@ -409,6 +409,35 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
             ConeF cf;
             cone_of(c, cf);
             pf.cone[j] = cf;
+            {
+                // extent of the silhouette edges below along and across the cone axis, about the
+                // centre the kernel forms (float32, 0.5f * (lo + hi)): the exact cull bounds every
+                // edge point by it (a slab across a nearly flat patch is far thinner than the sphere
+                // around it, which the quantized record falls back to)
+                float cen[3];
+                for (int k = 0; k < 3; ++k) cen[k] = 0.5f * (pf.lo[j][k] + pf.hi[j][k]);
+                const double u[3] = {cf.axis[0], cf.axis[1], cf.axis[2]};
+                double amin = 0.0, amax = 0.0, perp2 = 0.0;
+                bool first = true;
+                for (uint32_t s = sb[c]; s < sb[c] + sc[c]; ++s) {
+                    const float4 s0 = out.sils[3 * s], s1 = out.sils[3 * s + 1];
+                    const double pts[2][3] = {{s0.x, s0.y, s0.z}, {s0.w, s1.x, s1.y}};
+                    for (const auto &pt : pts) {
+                        double d[3], al = 0.0;
+                        for (int k = 0; k < 3; ++k) { d[k] = pt[k] - cen[k]; al += d[k] * u[k]; }
+                        double q2 = 0.0;
+                        for (int k = 0; k < 3; ++k) q2 += (d[k] - al * u[k]) * (d[k] - al * u[k]);
+                        amin = first ? al : std::min(amin, al);
+                        amax = first ? al : std::max(amax, al);
+                        perp2 = std::max(perp2, q2);
+                        first = false;
+                    }
+                }
+                constexpr float kInfF = std::numeric_limits<float>::infinity();
+                pf.sil_lo[j] = std::nextafter((float)amin, -kInfF);
+                pf.sil_hi[j] = std::nextafter((float)amax, kInfF);
+                pf.sil_perp[j] = std::nextafter((float)std::sqrt(perp2), kInfF);
+            }
             // int16 cone: the axis rounding error is folded into the angle
             double ax[3] = {cf.axis[0], cf.axis[1], cf.axis[2]};
             int16_t qa[3];

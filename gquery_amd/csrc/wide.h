@@ -92,7 +92,10 @@ template <int D> struct alignas(64) SPairF {
     ConeF cone[2];
     uint32_t child[2];
     int32_t ref_cone[2];    // index into the reference-cone table, -1 = the reference test cannot cull
-    uint8_t pad[128 - (4 * D * 4 + 2 * 20 + 16)];
+    // extent of the silhouette edges below about the child box centre c = 0.5f * (lo + hi), in the
+    // frame of the child's exact-cull cone axis u: every edge endpoint p has (p - c).u in
+    // [sil_lo, sil_hi] and lies within sil_perp of the line c + t u (rounded outward)
+    float sil_lo[2], sil_hi[2], sil_perp[2];
 };
 // 2D: the reference cone test can cull at nearly every node (its float32 half-angles are small),
 // and it needs the node's float32 box and cone exactly, so the 2D pair record carries both
@@ -129,7 +132,8 @@ static_assert(offsetof(SPairQ<3>, cone_axis) == 28, "cone axis");
 static_assert(offsetof(SPairQ<3>, cone_cs) == 40, "cone cs");
 static_assert(offsetof(SPairQ<3>, child) == 48, "pair child");
 static_assert(offsetof(SPairQ<3>, ref_cone) == 56, "pair ref");
-static_assert(offsetof(SPairF<3>, cone) == 48 && offsetof(SPairF<3>, child) == 88 && offsetof(SPairF<3>, ref_cone) == 96, "SPairF<3>");
+static_assert(offsetof(SPairF<3>, cone) == 48 && offsetof(SPairF<3>, child) == 88 && offsetof(SPairF<3>, ref_cone) == 96 &&
+              offsetof(SPairF<3>, sil_lo) == 104 && offsetof(SPairF<3>, sil_perp) == 120, "SPairF<3>");
 static_assert(offsetof(SPairF<2>, cone_axis) == 32 && offsetof(SPairF<2>, cone_half) == 48 &&
               offsetof(SPairF<2>, cone_radius) == 56 && offsetof(SPairF<2>, child) == 64 &&
               offsetof(SPairF<2>, ref_cone) == 72 && offsetof(SPairF<2>, cull) == 80 &&

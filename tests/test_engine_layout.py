@@ -36,7 +36,8 @@ def spf_dtype(D):
                          ("cull", "<i4", 2), ("sin", "<f4", 2), ("cos", "<f4", 2), ("pad", "u1", 24)])
     return np.dtype([("lo", "<f4", (2, D)), ("hi", "<f4", (2, D)), ("axis0", "<f4", 3), ("cos0", "<f4"),
                      ("sin0", "<f4"), ("axis1", "<f4", 3), ("cos1", "<f4"), ("sin1", "<f4"),
-                     ("child", "<u4", 2), ("ref", "<i4", 2), ("pad", "u1", 128 - (4 * D * 4 + 56))])
+                     ("child", "<u4", 2), ("ref", "<i4", 2), ("slo", "<f4", 2), ("shi", "<f4", 2),
+                     ("sperp", "<f4", 2)])
 
 
 def decode(q, e, origin):
@@ -198,6 +199,19 @@ def test_snch_pair_layout(name):
             if D == 3 and sl:
                 geo = seg[sl].reshape(-1, 3)
                 assert (geo >= lo).all() and (geo <= hi).all()
+                # the exact cull's extents: every edge endpoint below, relative to the float32 box
+                # centre, projects onto the cone axis within [slo, shi] and lies within sperp of the
+                # axis line; each bound is attained to within rounding
+                cen = (np.float32(0.5) * (lo + hi)).astype(np.float64)
+                ax = pf[i]["axis0" if j == 0 else "axis1"].astype(np.float64)
+                d = geo.astype(np.float64) - cen
+                al = d @ ax
+                pe = np.linalg.norm(d - al[:, None] * ax, axis=1)
+                slo, shi, sperp = (float(pf[i][k][j]) for k in ("slo", "shi", "sperp"))
+                scale = np.abs(d).max() + 1e-30
+                assert slo <= al.min() and al.max() <= shi and pe.max() <= sperp
+                assert al.min() - slo <= 1e-6 * scale and shi - al.max() <= 1e-6 * scale
+                assert sperp - pe.max() <= 1e-6 * scale
                 cos_t = pf[i]["cos0" if j == 0 else "cos1"]
                 if 0 < cos_t < 1.5:
                     ax = pf[i]["axis0" if j == 0 else "axis1"].astype(np.float64)
