@@ -130,8 +130,9 @@ typedef struct gq_scene_options {
                               frontier outgrows it is finished by a second pass with the full
                               frontier: results do not change */
     int32_t sil_pairs;     /* 3D silhouette pair records under GQ_METHOD_ENGINE: 0 = float32 (128 B
-                              per pair, default: tighter boxes, fewer nodes opened), 1 = 8-bit
-                              quantized (64 B per pair, half the memory).  Results do not change */
+                              per pair, default: exact boxes and the edges' extent for the exact
+                              cull, fewer nodes opened), 1 = 8-bit quantized (64 B per pair, half
+                              the memory).  Results do not change */
     int32_t reserved[3];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
