@@ -547,11 +547,12 @@ constexpr int kFrontMax = 8192;     // frontier entries at most (64 KB of LDS)
 // frontier 437) 40.7 ms at 512-768 entries vs 42.8 ms at 1,024 (fewer resident wavefronts).
 constexpr int kFrontSmallCp = 1024, kFrontSmallSil = 640;
 // Silhouette small-frontier pass: children whose key is within kSilNear x the round's smallest are
-// appended last, so they are expanded first.  On the workload above: 34.6 ms at 1.25 (34.7 at 1.1,
-// 35.8 at 1.5, 37.5 at 2.0; three or four groups no better; 34.5-34.9 ms with 512-896 frontier
-// entries) vs 40.1 ms with lane-order appends.  The same grouping in the closest-point pass was
-// slower (7.57-7.69 ms vs 7.47 ms).
-constexpr float kSilNear = 1.25f;
+// appended last, so they are expanded first.  On the workload above, first measured at 34.6 ms
+// with 1.25 (35.8 at 1.5, 37.5 at 2.0; three or four groups no better) vs 40.1 ms with lane-order
+// appends; with the tighter exact cull the optimum moved: 15.8 ms at 1.1 (15.6 at 1.05, 15.9 at
+// 1.02, 17.1 at 1.0, 16.8 at 1.25, 17.8 at 1.5; 20,480 triangles: 11.7-11.9 ms at 1.0-1.1).  The
+// same grouping in the closest-point pass was slower (7.57-7.69 ms vs 7.47 ms).
+constexpr float kSilNear = 1.1f;
 
 // Wave-wide minima through DPP lane moves (register to register, no LDS): swap within pairs and
 // quads, mirror half-rows and rows (every lane of a 16-lane row then holds the row's minimum), then
