@@ -95,3 +95,16 @@ def test_generator_draws_on_the_device(gpu):
         assert np.array_equal(got.cpu().numpy().astype(np.uint32), want[:, k])
     f = g.next_float32().cpu().numpy()
     assert np.array_equal(f, ((want[:, 3] >> np.uint32(9)) | np.uint32(0x3F800000)).view(np.float32) - np.float32(1))
+
+
+def test_zero_area_leaf_gives_no_sample(gpu):
+    """The kernel's handling of a leaf whose visible primitives all have zero area (no sample),
+    bit for bit against the oracle, beside a leaf with a real triangle."""
+    from gquery_amd.shapes.mesh import Mesh
+    V = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [0, 1, 0], [5, 5, 5], [6, 5, 5], [5, 6, 5]], np.float32)
+    F = np.array([[0, 1, 2], [4, 5, 6]], np.int32)
+    shape = Mesh(V, F, device="cuda:0")
+    x = np.repeat(np.array([[1.0, 0.0, 0.1]], np.float32), 2048, 0)
+    for R in (0.5, 100.0):
+        ref = _check(shape, 3, x, R, PCG32(len(x), initstate=5, device="cuda:0"))
+        assert (ref["hit"] == 0).all() if R == 0.5 else 0 < ref["hit"].mean() < 1

@@ -258,3 +258,23 @@ def test_edge_cases(dim):
     r0, s0 = O.sample_boundary(dim, s.bvh_node_data(), s.primitive_data(), np.zeros((0, dim), np.float32), 1.0,
                                np.zeros(0, np.uint64), np.zeros(0, np.uint64))
     assert len(r0["hit"]) == 0 and len(s0) == 0
+
+
+def test_zero_area_primitives_give_no_sample():
+    """A leaf whose visible primitives all have zero area yields no sample (the reference would
+    gather primitive -1); a visible primitive with area beside a degenerate one is always chosen."""
+    V = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [0, 1, 0], [5, 5, 5], [6, 5, 5], [5, 6, 5]], np.float32)
+    F = np.array([[0, 1, 2], [4, 5, 6]], np.int32)         # collinear (zero area), then a real triangle
+    s = ext.SNCH3D(V, F)
+    r, st, st2 = _sample(3, s, (1.0, 0.0, 0.1), 0.5, 256)   # only the degenerate one is in the ball
+    assert not r["hit"].any() and not r["pdf"].any()
+    # both in the ball: every sample is the real triangle with the enumerated pmf; walks that end
+    # in the degenerate one's leaf give none, as often as the enumeration says
+    n = 4000
+    r, _, _ = _sample(3, s, (1.0, 0.0, 0.1), 100.0, n)
+    exact = _exact_probabilities(3, s.bvh_node_data(), s.primitive_data(), (1.0, 0.0, 0.1), 100.0)
+    hit = r["hit"].astype(bool)
+    assert set(exact) == {1} and (r["prim_id"][hit] == 1).all()
+    assert np.allclose(r["pmf"][hit], exact[1], rtol=1e-6)
+    p = exact[1]
+    assert abs(hit.mean() - p) < 5 * np.sqrt(p * (1 - p) / n) + 1e-9
