@@ -839,6 +839,7 @@ struct PairChild {
     int32_t ref_cone;
     float half, radius;
     float sin_h, cos_h;   // 2D: sin and cos of the reference half-angle
+    float ref_axis[2];    // 2D: the reference cone axis (`axis` holds the exact-cull cone's)
     float sil_lo, sil_hi, sil_perp;   // 3D float32 records: extent of the silhouette edges below
                                       // along / across the cone axis about the box centre
 };
@@ -846,10 +847,11 @@ struct PairChild {
 template <int D, bool Q>
 __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairChild (&c)[2]) {
     if constexpr (D == 2) {
-        // SPairF<2> (128 B, both node formats): lo[2][2] @0, hi[2][2] @16, cone axis @32, half @48,
-        // radius @56, child @64, ref_cone @72, cull @80, sin(half) @88, cos(half) @96
-        Words<28> r;
-        r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 128);
+        // SPairF<2> (192 B, both node formats): lo[2][2] @0, hi[2][2] @16, cone axis @32, half @48,
+        // radius @56, child @64, ref_cone @72, cull @80, sin(half) @88, cos(half) @96, exact-cull
+        // cos @104, sin @112, axis @120, vertex extents along @136 / @144, across @152
+        Words<40> r;
+        r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 192);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
@@ -858,16 +860,25 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
                 c[j].hi[k] = r.f32(16 + 8 * j + 4 * k);
                 c[j].axis[k] = r.f32(32 + 8 * j + 4 * k);
             }
-            c[j].lo[2] = c[j].hi[2] = c[j].axis[2] = 0.0f;
+            c[j].lo[2] = c[j].hi[2] = 0.0f;
             c[j].half = r.f32(48 + 4 * j);
             c[j].radius = r.f32(56 + 4 * j);
             c[j].child = r.u32(64 + 4 * j);
             c[j].ref_cone = (int32_t)r.u32(72 + 4 * j);
-            c[j].cos_t = r.u32(80 + 4 * j) ? 2.0f : 0.0f;
-            c[j].sin_t = 1.0f;
             c[j].sin_h = r.f32(88 + 4 * j);
             c[j].cos_h = r.f32(96 + 4 * j);
-            c[j].sil_lo = c[j].sil_hi = c[j].sil_perp = 0.0f;
+            // the exact cull's cone replaces the reference cone axis in `axis` (the reference test
+            // reads its cone from ref_axis)
+            c[j].ref_axis[0] = c[j].axis[0];
+            c[j].ref_axis[1] = c[j].axis[1];
+            c[j].cos_t = r.f32(104 + 4 * j);   // 2 = no silhouette below
+            c[j].sin_t = r.f32(112 + 4 * j);
+            c[j].axis[0] = r.f32(120 + 8 * j);
+            c[j].axis[1] = r.f32(124 + 8 * j);
+            c[j].axis[2] = 0.0f;
+            c[j].sil_lo = r.f32(136 + 4 * j);
+            c[j].sil_hi = r.f32(144 + 4 * j);
+            c[j].sil_perp = r.f32(152 + 4 * j);
         }
     } else if constexpr (Q) {
         // SPairQ<3> (64 B): origin, exp | child boxes | cone_axis int16 @28 | cone_cs int16 @40 |
@@ -971,7 +982,17 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
             return sphere || slab;
         }
     } else {
-        return false;
+        // 2D: the same bounds with the vertex extents (every record of this form is float32)
+        const float w0 = 0.5f * (c.lo[0] + c.hi[0]) - xc[0], w1 = 0.5f * (c.lo[1] + c.hi[1]) - xc[1];
+        const float a = w0 * c.axis[0] + w1 * c.axis[1];
+        const float b = fabsf(w0 * c.axis[1] - w1 * c.axis[0]);
+        const float lo_u = c.sil_lo, hi_u = c.sil_hi, rho = c.sil_perp;
+        const float R = sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho);
+        const float m = 1e-5f * (fabsf(a) + b + R) + 1e-6f;
+        const bool sphere = (a * c.cos_t - b * c.sin_t > R + m) || (a * c.cos_t + b * c.sin_t < -(R + m));
+        const bool slab = ((a + lo_u) * c.cos_t - (b + rho) * c.sin_t > m) ||
+                          ((a + hi_u) * c.cos_t + (b + rho) * c.sin_t < -m);
+        return sphere || slab;
     }
 }
 
@@ -996,7 +1017,7 @@ __device__ __forceinline__ bool ref_cone_2d(const PairChild &c, V2 x, float r, f
     const V2 va = (lo + hi) / 2.0f - x;
     const float l2 = dot(va, va);
     const float il = rsqrtf(l2);
-    const float cs = fabsf(c.axis[0] * va.x + c.axis[1] * va.y) * il;
+    const float cs = fabsf(c.ref_axis[0] * va.x + c.ref_axis[1] * va.y) * il;
     if (cs <= c.sin_h - m) return true;
     if (cs >= c.sin_h + m) {
         const float l = l2 * il;
@@ -1010,7 +1031,7 @@ __device__ __forceinline__ bool ref_cone_2d(const PairChild &c, V2 x, float r, f
             }
         }
     }
-    return snch_overlaps(lo, hi, v2(c.axis[0], c.axis[1]), c.half, c.radius, x, r, d_min);
+    return snch_overlaps(lo, hi, v2(c.ref_axis[0], c.ref_axis[1]), c.half, c.radius, x, r, d_min);
 }
 
 template <int D>

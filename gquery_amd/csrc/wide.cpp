@@ -259,6 +259,17 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
             out.sil_ids.push_back((int32_t)r[9]);
             nrm[s].ok = !open_end;
             nrm[s].always = false;
+            // the vertex predicate cross(b - a, x - a) * cross(c - b, x - b) < 0 is, with
+            // n = perp(segment) = (-dy, dx), dot(n1, x - b) * dot(n2, x - b): the same form as the 3D
+            // edge predicate, so the same cones bound it (z = 0)
+            float n1[3] = {-(r[3] - r[1]), r[2] - r[0], 0.0f}, n2[3] = {-(r[5] - r[3]), r[4] - r[2], 0.0f};
+            fnormalize(n1);
+            fnormalize(n2);
+            for (int k = 0; k < 3; ++k) {
+                nrm[s].n0[k] = n1[k];
+                nrm[s].n1[k] = n2[k];
+                nrm[s].ok = nrm[s].ok && std::isfinite(n1[k]) && std::isfinite(n2[k]);
+            }
         }
     }
     // subtree silhouette ranges (contiguous in leaf order)
@@ -283,7 +294,6 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
         }
         if (!any) { cf.cos_t = 2.0f; cf.sin_t = 0.0f; return; }  // no possible silhouette below: always cull
         if (always) return;                                        // a boundary edge below: never cull
-        if (D != 3) return;
         // orient each (n0, n1) pair along a common direction (the predicate is invariant under
         // flipping both), then bound all of them by one cone
         bool have_ref = false;
@@ -358,6 +368,29 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 ConeF cf;
                 cone_of(c, cf);
                 pf.cull[j] = cf.cos_t > 1.5f ? 1 : 0;
+                pf.ecos[j] = cf.cos_t;
+                pf.esin[j] = cf.sin_t;
+                pf.eaxis[j][0] = cf.axis[0];
+                pf.eaxis[j][1] = cf.axis[1];
+                // extent of the silhouette vertices below about the centre the kernel forms
+                const float cen[2] = {0.5f * (pf.lo[j][0] + pf.hi[j][0]), 0.5f * (pf.lo[j][1] + pf.hi[j][1])};
+                const double u[2] = {cf.axis[0], cf.axis[1]};
+                double amin = 0.0, amax = 0.0, perp = 0.0;
+                bool first = true;
+                for (uint32_t s = sb[c]; s < sb[c] + sc[c]; ++s) {
+                    const float4 v = out.sils[2 * s];   // (a.x, a.y, b.x, b.y): the vertex is b
+                    const double d[2] = {(double)v.z - cen[0], (double)v.w - cen[1]};
+                    const double al = d[0] * u[0] + d[1] * u[1];
+                    const double q = std::fabs(d[0] * u[1] - d[1] * u[0]);
+                    amin = first ? al : std::min(amin, al);
+                    amax = first ? al : std::max(amax, al);
+                    perp = std::max(perp, q);
+                    first = false;
+                }
+                constexpr float kInfF = std::numeric_limits<float>::infinity();
+                pf.elo[j] = std::nextafter((float)amin, -kInfF);
+                pf.ehi[j] = std::nextafter((float)amax, kInfF);
+                pf.eperp[j] = std::nextafter((float)perp, kInfF);
             }
             std::memcpy(out.snodes.data() + (size_t)pidx[i] * sizeof(SPairF<2>), &pf, sizeof pf);
         }

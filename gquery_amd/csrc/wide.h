@@ -8,7 +8,7 @@
 //      FNode<D> 128 B: child boxes in float32.
 //  * SNCH pairs: one record per binary interior node holding both children (box, reference
 //    normal-cone index, exact-cull cone, child reference) so a visit is one record fetch.
-//      SPairQ<D> 64 B (quantized boxes / cones), SPairF<D> 128 B (float32).
+//      SPairQ<3> 64 B (quantized boxes / cones), SPairF<D> 128 B in 3D / 192 B in 2D (float32).
 //  * primitives in leaf order as float4 records (3D: a, b, c; 2D: a|b), silhouettes in leaf
 //    order (3D: a, b, n0, n1 with the predicate normals precomputed; 2D: a|b, c).
 // Child references: bit 31 set = leaf, bits 24..30 = count, bits 0..23 = first primitive /
@@ -99,8 +99,10 @@ template <int D> struct alignas(64) SPairF {
 };
 // 2D: the reference cone test can cull at nearly every node (its float32 half-angles are small),
 // and it needs the node's float32 box and cone exactly, so the 2D pair record carries both
-// children's reference boxes and cones inline: one 128-byte record per visit instead of a pair
-// record plus two reference-cone fetches.  2D silhouettes use this record for both node formats.
+// children's reference boxes and cones inline: one record per visit instead of a pair record plus
+// two reference-cone fetches.  It also carries each child's exact-cull cone (over the vertex
+// predicate's segment normals) and the extent of the silhouette vertices below along and across
+// its axis.  2D silhouettes use this record for both node formats.
 template <> struct alignas(64) SPairF<2> {
     float lo[2][2], hi[2][2];            // the children's boxes (SNCH node data, float32)
     float cone_axis[2][2];               // reference cone axis of each child
@@ -109,9 +111,13 @@ template <> struct alignas(64) SPairF<2> {
     int32_t ref_cone[2];                 // 0 = the reference cone test can cull, -1 = it cannot
     int32_t cull[2];                     // 1 = no silhouette below this child (always skipped)
     float cone_sin[2], cone_cos[2];      // sin and cos of the half-angle (the kernels' fast path)
-    uint8_t pad[24];
+    float ecos[2], esin[2];              // exact-cull cone: cos / sin of its angle (cos 2 = no
+                                         // silhouette below, cos <= 0 = never culls)
+    float eaxis[2][2];                   // exact-cull cone axis
+    float elo[2], ehi[2], eperp[2];      // vertices below about the box centre: along / across the axis
+    uint8_t pad[32];
 };
-static_assert(sizeof(SPairF<3>) == 128 && sizeof(SPairF<2>) == 128, "SPairF is two lines");
+static_assert(sizeof(SPairF<3>) == 128 && sizeof(SPairF<2>) == 192, "SPairF: two lines in 3D, three in 2D");
 
 // SNCH pair record (3D), 8-bit quantized boxes and int16 cones.
 template <int D> struct SPairQ;
@@ -137,7 +143,10 @@ static_assert(offsetof(SPairF<3>, cone) == 48 && offsetof(SPairF<3>, child) == 8
 static_assert(offsetof(SPairF<2>, cone_axis) == 32 && offsetof(SPairF<2>, cone_half) == 48 &&
               offsetof(SPairF<2>, cone_radius) == 56 && offsetof(SPairF<2>, child) == 64 &&
               offsetof(SPairF<2>, ref_cone) == 72 && offsetof(SPairF<2>, cull) == 80 &&
-              offsetof(SPairF<2>, cone_sin) == 88 && offsetof(SPairF<2>, cone_cos) == 96, "SPairF<2>");
+              offsetof(SPairF<2>, cone_sin) == 88 && offsetof(SPairF<2>, cone_cos) == 96 &&
+              offsetof(SPairF<2>, ecos) == 104 && offsetof(SPairF<2>, esin) == 112 &&
+              offsetof(SPairF<2>, eaxis) == 120 && offsetof(SPairF<2>, elo) == 136 &&
+              offsetof(SPairF<2>, ehi) == 144 && offsetof(SPairF<2>, eperp) == 152, "SPairF<2>");
 static_assert(offsetof(QNode<3>, qlo) == 16 && offsetof(QNode<3>, qhi) == 28 && offsetof(QNode<3>, child) == 40, "QNode<3>");
 static_assert(offsetof(QNode<2>, nchild) == 10 && offsetof(QNode<2>, qlo) == 12 && offsetof(QNode<2>, qhi) == 28 &&
               offsetof(QNode<2>, child) == 44, "QNode<2>");

@@ -33,7 +33,9 @@ def spf_dtype(D):
     if D == 2:
         return np.dtype([("lo", "<f4", (2, 2)), ("hi", "<f4", (2, 2)), ("cone_axis", "<f4", (2, 2)),
                          ("half", "<f4", 2), ("radius", "<f4", 2), ("child", "<u4", 2), ("ref", "<i4", 2),
-                         ("cull", "<i4", 2), ("sin", "<f4", 2), ("cos", "<f4", 2), ("pad", "u1", 24)])
+                         ("cull", "<i4", 2), ("sin", "<f4", 2), ("cos", "<f4", 2), ("ecos", "<f4", 2),
+                         ("esin", "<f4", 2), ("eaxis", "<f4", (2, 2)), ("elo", "<f4", 2), ("ehi", "<f4", 2),
+                         ("eperp", "<f4", 2), ("pad", "u1", 32)])
     return np.dtype([("lo", "<f4", (2, D)), ("hi", "<f4", (2, D)), ("axis0", "<f4", 3), ("cos0", "<f4"),
                      ("sin0", "<f4"), ("axis1", "<f4", 3), ("cos1", "<f4"), ("sin1", "<f4"),
                      ("child", "<u4", 2), ("ref", "<i4", 2), ("slo", "<f4", 2), ("shi", "<f4", 2),
@@ -192,6 +194,25 @@ def test_snch_pair_layout(name):
                 assert pf[i]["ref"][j] == (0 if 0.0 <= half <= np.float32(np.pi / 2) else -1)
                 assert abs(pf[i]["sin"][j] - np.sin(np.float64(half))) <= 1e-7
                 assert abs(pf[i]["cos"][j] - np.cos(np.float64(half))) <= 1e-7
+                # exact-cull cone over the vertex predicate's segment normals, and the vertices'
+                # extent along / across its axis about the float32 box centre
+                ecos = pf[i]["ecos"][j]
+                assert (ecos > 1.5) == bool(pf[i]["cull"][j])
+                if 0 < ecos < 1.5 and sl:
+                    ax = pf[i]["eaxis"][j].astype(np.float64)
+                    A, B, Cc = (sils[sl, k:k + 2].astype(np.float64) for k in (0, 2, 4))
+                    n1 = np.stack([-(B - A)[:, 1], (B - A)[:, 0]], 1)
+                    n2 = np.stack([-(Cc - B)[:, 1], (Cc - B)[:, 0]], 1)
+                    ok = np.isfinite(n1).all(1) & np.isfinite(n2).all(1)
+                    c1 = (n1[ok] @ ax) / np.linalg.norm(n1[ok], axis=1)
+                    c2 = (n2[ok] @ ax) / np.linalg.norm(n2[ok], axis=1)
+                    sg = np.sign(c1 + c2)
+                    assert (sg * c1 >= ecos - 1e-6).all() and (sg * c2 >= ecos - 1e-6).all()
+                    cen = (np.float32(0.5) * (lo + hi)).astype(np.float64)
+                    d = B - cen
+                    al = d @ ax
+                    pe = np.abs(d[:, 0] * ax[1] - d[:, 1] * ax[0])
+                    assert pf[i]["elo"][j] <= al.min() and al.max() <= pf[i]["ehi"][j] and pe.max() <= pf[i]["eperp"][j]
                 continue
             qlo = decode(pq[i]["qlo"][j], pq[i]["exp"], pq[i]["origin"])
             qhi = decode(pq[i]["qhi"][j], pq[i]["exp"], pq[i]["origin"])
