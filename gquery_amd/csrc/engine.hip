@@ -11,8 +11,9 @@
 // Pruning is conservative: a node is skipped only when no primitive in it can beat the current
 // answer, so results equal the reference traversal's up to exact ties (broken to the lowest
 // primitive slot everywhere, so a query's result does not depend on its neighbours):
-//  * closest point: squared box distance against best^2 (inflated by 2e-6 relative so float
-//    rounding never drops a box holding an equally near primitive);
+//  * closest point: squared box distance against (best (1 + 1e-6) + slack)^2, where the slack
+//    scales with the coordinates, so float rounding never drops a box holding a primitive whose
+//    float32 distance is equal or nearer;
 //  * ray: slab test with a reciprocal direction widened so it never drops a box the reference's
 //    division-based test keeps;
 //  * silhouette: the reference's box and normal-cone tests (the cone test is evaluated with the
@@ -56,6 +57,15 @@ constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its o
 constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil3 = 128, kPacketBudgetSil2 = 750;
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
+// Absolute rounding slack per unit of coordinate magnitude: 32 float32 ulps of 1.  The float32
+// distance to a triangle or segment rebuilds the nearest point in absolute coordinates, so its
+// rounding error grows with |x| and the scene's coordinates, not with the distance itself; every
+// pruning test that must never drop a primitive the float32 leaf test would accept widens its bound
+// by kAbsSlack * (|x|_1 + S_1), where S_1 bounds |p|_1 over the scene (from its root box).  The
+// error terms: the prefilter's fma chain and stored plane (4 ulps of |x|_1 + S_1), the nearest
+// point's reconstruction (12 ulps of S_1), the exact cull's view vector and its cross product
+// (about 20 ulps), each well inside 32.
+constexpr float kAbsSlack = 32.0f * 5.9604644775390625e-08f;
 
 struct EngineParams {
     const void *nodes;        // BVH4 (FNode / QNode) or SNCH pairs (SPairF / SPairQ)
@@ -165,6 +175,20 @@ __device__ __forceinline__ GP gview(const EngineParams &in) {
     P.out.prim_id = gcast(in.out.prim_id);
     P.out.on_boundary = gcast(in.out.on_boundary);
     return P;
+}
+
+// Absolute rounding slack of a query at xc against the job's scene (kAbsSlack above).
+template <int D> __device__ __forceinline__ float coord_slack(const GP &P, const float *xc) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) s += fabsf(xc[k]) + fmaxf(fabsf(P.root_lo[k]), fabsf(P.root_hi[k]));
+    return kAbsSlack * s;
+}
+// Squared pruning bound for a closest-point answer at distance d: boxes farther than this cannot
+// hold a primitive whose float32 distance is below or equal to d.
+__device__ __forceinline__ float cp_bound2(float d, float slack) {
+    const float b = fmaf(d, 1.000001f, slack);
+    return b * b;
 }
 
 // ---- lane-per-query stacks.  In LDS: depth chosen per launch from the tree depth, one column per
@@ -432,6 +456,7 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
 #pragma unroll
     for (int k = 0; k < D; ++k) pc[k] = comp<D>(p, k);
     packet_centroid<D>(live, pc, cen);
+    const float slack = coord_slack<D>(P, pc);
     float best = kInf, bound2 = live ? kInf : -1.0f;   // a dead lane never opens anything
     int best_slot = -1;
     int sp = 0;
@@ -504,10 +529,10 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
                 if constexpr (D == 3) {
                     const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
                     const float4 pl = ldg(P.prims + (4 * s + 3));
-                    // plane-distance lower bound, widened for rounding: skip the exact test when no
-                    // lane can get strictly closer through this triangle
+                    // plane-distance lower bound, widened for rounding (relative and absolute):
+                    // skip the exact test when no lane can get closer through this triangle
                     const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
-                    if (__ballot(lb * 0.99999f - 1e-6f < best) == 0ull) continue;
+                    if (__ballot(fmaf(lb, 0.99999f, -slack) <= best) == 0ull) continue;
                     V3 pt;
                     V2 uv;
                     dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
@@ -519,7 +544,7 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
                 if (live && (dd < best || (dd == best && (int)s < best_slot))) {
                     best = dd;
                     best_slot = (int)s;
-                    bound2 = dd * dd * 1.000002f;
+                    bound2 = cp_bound2(dd, slack);
                 }
             }
         }
@@ -620,7 +645,8 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
         float pc[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) pc[j] = comp<D>(p, j);
-        float bound2 = best < kInf ? best * best * 1.000002f : kInf;
+        const float slack = coord_slack<D>(P, pc);
+        float bound2 = best < kInf ? cp_bound2(best, slack) : kInf;
         int size = 1;
         bool overflow = false;
         front[0] = make_uint2(P.root, 0u);
@@ -656,7 +682,7 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                         if constexpr (D == 3) {
                             const float4 pl = ldg(P.prims + (4 * s + 3));
                             const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
-                            if (lb * 0.99999f - 1e-6f >= best) continue;
+                            if (fmaf(lb, 0.99999f, -slack) > best) continue;
                             const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
                             V3 pt;
                             V2 uv;
@@ -678,7 +704,7 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
             if (cand < best || (cand == best && cand_slot < best_slot)) {
                 best = cand;
                 best_slot = cand_slot;
-                bound2 = cand * cand * 1.000002f;
+                bound2 = cp_bound2(cand, slack);
             }
             // append the opened children, in lane order and child order within a lane
             int tot = 0, pos = size;
@@ -944,9 +970,10 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
 
 // True when every silhouette predicate in the child is provably false for query x: the normals
 // lie within the cone and the dot product of every view direction into the box with every cone
-// normal has one sign (bounded away from zero by a margin covering float32 rounding).
+// normal has one sign (bounded away from zero by a margin covering float32 rounding: 1e-5 of the
+// terms plus mabs, the query's absolute coordinate slack).
 template <int D, bool Q>
-__device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) {
+__device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc, float mabs) {
     if (c.cos_t > 1.5f) return true;     // no silhouette can ever pass in this subtree
     if (!(c.cos_t > 0.0f)) return false; // cone too wide to separate anything
     if constexpr (D == 3) {
@@ -965,7 +992,7 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
             // quantized records: a sphere of the box's half-diagonal R around the centre bounds
             // every edge point p, so dot(p - x, n) >= a cos - b sin - R for every cone normal n
             const float R = sqrtf(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
-            const float m = R + 1e-5f * (fabsf(a) + b + R) + 1e-6f;
+            const float m = R + 1e-5f * (fabsf(a) + b + R) + mabs;
             return (a * c.cos_t - b * c.sin_t > m) || (a * c.cos_t + b * c.sin_t < -m);
         } else {
             // float32 records: the edges' extent along the axis, [lo_u, hi_u], and across it, rho.
@@ -975,7 +1002,7 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
             // bound above; either proof culls
             const float lo_u = c.sil_lo, hi_u = c.sil_hi, rho = c.sil_perp;
             const float R = sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho);
-            const float m = 1e-5f * (fabsf(a) + b + R) + 1e-6f;
+            const float m = 1e-5f * (fabsf(a) + b + R) + mabs;
             const bool sphere = (a * c.cos_t - b * c.sin_t > R + m) || (a * c.cos_t + b * c.sin_t < -(R + m));
             const bool slab = ((a + lo_u) * c.cos_t - (b + rho) * c.sin_t > m) ||
                               ((a + hi_u) * c.cos_t + (b + rho) * c.sin_t < -m);
@@ -988,7 +1015,7 @@ __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc) 
         const float b = fabsf(w0 * c.axis[1] - w1 * c.axis[0]);
         const float lo_u = c.sil_lo, hi_u = c.sil_hi, rho = c.sil_perp;
         const float R = sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho);
-        const float m = 1e-5f * (fabsf(a) + b + R) + 1e-6f;
+        const float m = 1e-5f * (fabsf(a) + b + R) + mabs;
         const bool sphere = (a * c.cos_t - b * c.sin_t > R + m) || (a * c.cos_t + b * c.sin_t < -(R + m));
         const bool slab = ((a + lo_u) * c.cos_t - (b + rho) * c.sin_t > m) ||
                           ((a + hi_u) * c.cos_t + (b + rho) * c.sin_t < -m);
@@ -1063,6 +1090,7 @@ __device__ __forceinline__ void sil_packet(const GP &P, int base) {
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = comp<D>(x, k);
     packet_centroid<D>(live, xc, cen);
+    const float cslack = coord_slack<D>(P, xc);
     float r = live ? (P.r_max ? P.r_max[qi] : P.r_max_scalar) : -1.0f;
     bool found = false;
     V best_p{};
@@ -1099,7 +1127,7 @@ __device__ __forceinline__ void sil_packet(const GP &P, int base) {
                     box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                     bool h = act && dm <= r;
                     if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
-                    if (h && exact_cull<D, Q>(c[j], xc)) h = false;
+                    if (h && exact_cull<D, Q>(c[j], xc, cslack)) h = false;
                     hh[j] = h;
                     hm[j] = __ballot(h);
                     any[j] = hm[j] != 0ull;
@@ -1235,6 +1263,7 @@ __device__ __forceinline__ void sil_lane(const GP &P, int base) {
     float xc[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int k = 0; k < D; ++k) xc[k] = comp<D>(x, k);
+    const float cslack = coord_slack<D>(P, xc);
     float r = P.r_max ? P.r_max[qi] : P.r_max_scalar;
     int best_slot = -1;
     float d_root, d_unused;
@@ -1256,7 +1285,7 @@ __device__ __forceinline__ void sil_lane(const GP &P, int base) {
                     float dr;
                     hj = ref_cone_pass<D>(P, c[j], x, r, dr);
                 }
-                if (hj && exact_cull<D, Q>(c[j], xc)) hj = false;
+                if (hj && exact_cull<D, Q>(c[j], xc, cslack)) hj = false;
                 h[j] = hj;
                 key[j] = dm;
             }
@@ -1321,6 +1350,7 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
         float xc[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int j = 0; j < D; ++j) xc[j] = comp<D>(x, j);
+        const float cslack = coord_slack<D>(P, xc);
         int size = 0;
         bool overflow = false;
         if (r >= 0.0f) {
@@ -1347,7 +1377,7 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
                         box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                         bool h = dm <= r;
                         if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
-                        if (h && exact_cull<D, Q>(c[j], xc)) h = false;
+                        if (h && exact_cull<D, Q>(c[j], xc, cslack)) h = false;
                         const uint2 ent = make_uint2(c[j].child, __float_as_uint(dm));
                         if (j == 0) { h0 = h; ch0 = ent; }
                         else { h1 = h; ch1 = ent; }

@@ -24,6 +24,8 @@
  * primitive and runner-up distance (closest point), first and second hit (rays), nearest valid
  * silhouette and runner-up (silhouettes).  These pin the FP32 results to the mathematics.
  *
+ * Part 2b (FP32): brute force with the traversal's own leaf test (exact target for conservative
+ * pruning at any coordinate scale), and the leaf distance of point/triangle pairs.
  * Part 3 (FP32): the brute-force *_baseline queries and mesh normals (gquery/shapes/polyline.py,
  * mesh.py).  Part 4: BVH-guided boundary sampling (gquery/shapes/bvh3d.py:246-346) and the PCG32
  * generator it draws from.
@@ -639,12 +641,12 @@ void or_bf_closest_point_f64(int D, const float *V, const int32_t *F, int64_t nf
 /* First ray hit by brute force in double (exact Moller-Trumbore / 2D cross test, no epsilon):
  * hit distance, id and the second-smallest hit distance. */
 void or_bf_intersect_f64(int D, const float *V, const int32_t *F, int64_t nf, const float *x, const float *vdir,
-                         int64_t n, double *best_t, int32_t *best_id, double *second_t) {
+                         int64_t n, double *best_t, int32_t *best_id, double *second_t, double *best_uv) {
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t i = 0; i < n; ++i) {
         double o[3], v[3], a[3], b[3], c[3];
         for (int k = 0; k < D; ++k) { o[k] = x[D * i + k]; v[k] = vdir[D * i + k]; }
-        double t1 = INFINITY, t2 = INFINITY;
+        double t1 = INFINITY, t2 = INFINITY, uv1[2] = {0.0, 0.0}, uv[2] = {0.0, 0.0};
         int32_t id = -1;
         for (int64_t f = 0; f < nf; ++f) {
             double t = INFINITY;
@@ -676,15 +678,18 @@ void or_bf_intersect_f64(int D, const float *V, const int32_t *F, int64_t nf, co
                         double ww = dotd(3, v, q) * inv;
                         if (ww >= 0 && uu + ww <= 1) {
                             double tt = dotd(3, e2, q) * inv;
-                            if (tt >= 0) t = tt;
+                            /* barycentrics of the hit as Triangle.ray_intersect reports them
+                             * (triangle.py:69): (1 - v - w, v) */
+                            if (tt >= 0) { t = tt; uv[0] = 1.0 - uu - ww; uv[1] = uu; }
                         }
                     }
                 }
             }
-            if (t < t1) { t2 = t1; t1 = t; id = (int32_t)f; }
+            if (t < t1) { t2 = t1; t1 = t; id = (int32_t)f; uv1[0] = uv[0]; uv1[1] = uv[1]; }
             else if (t < t2) t2 = t;
         }
         best_t[i] = t1; best_id[i] = id; second_t[i] = t2;
+        if (best_uv) { best_uv[2 * i] = uv1[0]; best_uv[2 * i + 1] = uv1[1]; }
     }
 }
 
@@ -727,6 +732,48 @@ void or_bf_silhouette_f64(int D, const float *raw, int64_t ns, const float *x, i
         }
         best_d[i] = b1; second_d[i] = b2; best_id[i] = id;
         for (int k = 0; k < D; ++k) best_p[D * i + k] = bp[k];
+    }
+}
+
+/* ---- Part 2b (FP32): brute force with the traversal's own leaf test ----------------------
+ * The smallest float32 leaf distance over every primitive of the reference flat array (stride
+ * 5 segments / 10 triangles, the leaf tests of bvh.py:156-160 / bvh3d.py:123-128, math.py:14-21 /
+ * math.py:42-107).  A traversal whose pruning never drops a primitive the float32 leaf test would
+ * accept returns exactly this distance, bit for bit, at any coordinate scale. */
+void or_bf_closest_point_f32(int D, const float *prims, int64_t np, const float *x, int64_t n, float *best_d,
+                             int32_t *best_id) {
+    const int PS = D == 2 ? 5 : 10;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        const float *p = x + D * i;
+        float b = INFINITY, pt[3], uv[2];
+        int32_t id = -1;
+        for (int64_t j = 0; j < np; ++j) {
+            const float *pr = prims + PS * j;
+            const float dd = D == 3 ? cp_triangle(p, pr, pr + 3, pr + 6, pt, uv) : cp_segment(2, p, pr, pr + 2, pt);
+            if (dd < b) { b = dd; id = (int32_t)pr[PS - 1]; }
+        }
+        best_d[i] = b;
+        best_id[i] = id;
+    }
+}
+
+/* The float32 leaf distance from query i to the primitive in row rows[i] of the flat array. */
+void or_leaf_distance(int D, const float *prims, const int64_t *rows, const float *x, int64_t n, float *d) {
+    const int PS = D == 2 ? 5 : 10;
+    for (int64_t i = 0; i < n; ++i) {
+        const float *pr = prims + PS * rows[i];
+        float pt[3], uv[2];
+        d[i] = D == 3 ? cp_triangle(x + 3 * i, pr, pr + 3, pr + 6, pt, uv) : cp_segment(2, x + 2 * i, pr, pr + 2, pt);
+    }
+}
+
+/* The float32 leaf distance of n (point, triangle) pairs (closest_point_triangle, math.py:42-107):
+ * p[3n], tri[9n] (a, b, c).  For checks of lower bounds against what the leaf test computes. */
+void or_cp_triangle_pairs(const float *p, const float *tri, int64_t n, float *d) {
+    for (int64_t i = 0; i < n; ++i) {
+        float pt[3], uv[2];
+        d[i] = cp_triangle(p + 3 * i, tri + 9 * i, tri + 9 * i + 3, tri + 9 * i + 6, pt, uv);
     }
 }
 

@@ -48,8 +48,11 @@ def lib():
         for f in (h.or_closest_point, h.or_intersect, h.or_silhouette):
             f.restype = i64
         h.or_bf_closest_point_f64.argtypes = [C.c_int, vp, vp, i64, vp, i64, vp, vp, vp, vp]
-        h.or_bf_intersect_f64.argtypes = [C.c_int, vp, vp, i64, vp, vp, i64, vp, vp, vp]
+        h.or_bf_intersect_f64.argtypes = [C.c_int, vp, vp, i64, vp, vp, i64, vp, vp, vp, vp]
         h.or_bf_silhouette_f64.argtypes = [C.c_int, vp, i64, vp, i64, vp, vp, vp, vp]
+        h.or_bf_closest_point_f32.argtypes = [C.c_int, vp, i64, vp, i64, vp, vp]
+        h.or_cp_triangle_pairs.argtypes = [vp, vp, i64, vp]
+        h.or_leaf_distance.argtypes = [C.c_int, vp, vp, vp, i64, vp]
         h.or_num_threads.restype = C.c_int
         h.or_normals.argtypes = [vp, i64, vp, i64, vp, vp]
         h.or_cp_baseline.argtypes = [C.c_int, vp, vp, i64, vp, vp, i64, _Out]
@@ -74,13 +77,14 @@ def _records(n, dim, kind):
         "visits": np.zeros(n, np.int32), "child_tests": np.zeros(n, np.int32), "leaf_tests": np.zeros(n, np.int32),
     }
     if kind != "sil":
+        # the records' field names: uv (3D barycentrics), t (2D segment parameter)
         r["n"] = np.zeros((n, dim), np.float32)
-        r["uv"] = np.zeros((n, 2), np.float32) if dim == 3 else np.zeros(n, np.float32)
+        r["uv" if dim == 3 else "t"] = np.zeros((n, 2), np.float32) if dim == 3 else np.zeros(n, np.float32)
     if kind == "ray":
         r["on_boundary"] = np.zeros(n, np.uint8)
     o = _Out()
     for k, v in r.items():
-        setattr(o, k, v.ctypes.data)
+        setattr(o, "uv" if k == "t" else k, v.ctypes.data)
     return r, o
 
 
@@ -133,10 +137,13 @@ def bf_closest_point(dim, V, F, x):
 def bf_intersect(dim, V, F, x, v):
     V, F, x, v = _c(V, np.float32), _c(F, np.int32), _c(x, np.float32), _c(v, np.float32)
     n = x.shape[0]
-    t, i, t2 = np.zeros(n), np.zeros(n, np.int32), np.zeros(n)
+    t, i, t2, uv = np.zeros(n), np.zeros(n, np.int32), np.zeros(n), np.zeros((n, 2))
     lib().or_bf_intersect_f64(dim, V.ctypes.data, F.ctypes.data, F.shape[0], x.ctypes.data, v.ctypes.data, n,
-                              t.ctypes.data, i.ctypes.data, t2.ctypes.data)
-    return {"t": t, "prim_id": i, "second_t": t2}
+                              t.ctypes.data, i.ctypes.data, t2.ctypes.data, uv.ctypes.data)
+    out = {"t": t, "prim_id": i, "second_t": t2}
+    if dim == 3:
+        out["uv"] = uv   # barycentrics (1 - v - w, v) of the first hit
+    return out
 
 
 def bf_silhouette(dim, raw_silhouette_data, x):
@@ -147,6 +154,39 @@ def bf_silhouette(dim, raw_silhouette_data, x):
     lib().or_bf_silhouette_f64(dim, raw.ctypes.data, raw.size // ss, x.ctypes.data, n,
                                d.ctypes.data, p.ctypes.data, i.ctypes.data, d2.ctypes.data)
     return {"d": d, "p": p, "prim_id": i, "second_d": d2}
+
+
+def bf_closest_point_f32(dim, primitive_data, x):
+    """Smallest float32 leaf distance over every primitive (the traversal's own leaf formula) and
+    the index of the first primitive reaching it."""
+    prims, x = _c(primitive_data, np.float32), _c(x, np.float32)
+    n = x.shape[0]
+    d, i = np.zeros(n, np.float32), np.zeros(n, np.int32)
+    lib().or_bf_closest_point_f32(dim, prims.ctypes.data, prims.size // (5 if dim == 2 else 10),
+                                  x.ctypes.data, n, d.ctypes.data, i.ctypes.data)
+    return {"d": d, "prim_id": i}
+
+
+def leaf_distance(dim, primitive_data, prim_id, x):
+    """float32 leaf distance (the traversal's own formula) from each query to the primitive whose
+    original index is prim_id[i]."""
+    ps = 5 if dim == 2 else 10
+    prims, x = _c(primitive_data, np.float32), _c(x, np.float32)
+    idx = prims.reshape(-1, ps)[:, -1].astype(np.int64)
+    row_of = np.empty(idx.size, np.int64)
+    row_of[idx] = np.arange(idx.size)
+    rows = _c(row_of[np.asarray(prim_id, np.int64)], np.int64)
+    d = np.zeros(x.shape[0], np.float32)
+    lib().or_leaf_distance(dim, prims.ctypes.data, rows.ctypes.data, x.ctypes.data, x.shape[0], d.ctypes.data)
+    return d
+
+
+def cp_triangle_pairs(p, tri):
+    """float32 closest_point_triangle distance for point/triangle pairs: p (n, 3), tri (n, 3, 3)."""
+    p, tri = _c(p, np.float32), _c(tri, np.float32)
+    d = np.zeros(p.shape[0], np.float32)
+    lib().or_cp_triangle_pairs(p.ctypes.data, tri.ctypes.data, p.shape[0], d.ctypes.data)
+    return d
 
 
 def num_threads() -> int:

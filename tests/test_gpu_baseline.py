@@ -12,7 +12,7 @@ from tests import parity
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("valid", "p", "n", "uv", "d", "prim_id", "on_boundary")
+FIELDS = ("valid", "p", "n", "uv", "t", "d", "prim_id", "on_boundary")
 
 
 def _np(r):

@@ -9,7 +9,7 @@ from tests.test_gpu_parity import scene
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("valid", "p", "n", "uv", "d", "prim_id", "on_boundary")
+FIELDS = ("valid", "p", "n", "uv", "t", "d", "prim_id", "on_boundary")
 
 
 def _np(r):

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 N_FULL = 1 << 20
 
 
-def _np(r, fields=("valid", "prim_id", "d", "p", "n", "uv", "on_boundary")):
+def _np(r, fields=("valid", "prim_id", "d", "p", "n", "uv", "t", "on_boundary")):
     torch.cuda.synchronize()
     return {f: getattr(r, f).cpu().numpy() for f in fields if hasattr(r, f)}
 
@@ -69,8 +69,8 @@ def test_config1_star1024_full(gpu, method):
             ref = O.intersect(2, s.bvh_node_data(), s.primitive_data(), x, v)
             b = O.bf_intersect(2, V, F, x, v)
             gap = parity.gap_mask(b["t"], b["second_t"])
-        res = parity.summarize(f"config1/{kind}/{method}", got, ref, gap, vec_fields=("p", "n"))
-        assert res["ok"], res
+        res = parity.summarize(f"config1/{kind}/{method}", got, ref, gap)
+        assert res["ok"], parity.brief(res)
         assert res["id_checked"] > n // 4
 
 
@@ -102,8 +102,8 @@ def test_config2_icosphere_s6_full(gpu):
     ref = O.closest_point(3, s.bvh_node_data(), s.primitive_data(), x[sample])
     b = O.bf_closest_point(3, V, F, x[sample])
     got = {k: a[sample] for k, a in eng.items()}
-    res = parity.summarize("config2/cp", got, ref, parity.gap_mask(b["d"], b["second_d"]), vec_fields=("p", "n"))
-    assert res["ok"], res
+    res = parity.summarize("config2/cp", got, ref, parity.gap_mask(b["d"], b["second_d"]))
+    assert res["ok"], parity.brief(res)
     assert parity.rel_err(got["d"], b["d"]).max() <= parity.TOL_F64
 
     # rays: engine vs plain on every ray (the engine's FP64 band can only make it closer to FP64)
@@ -157,3 +157,46 @@ def test_config3_silhouettes_full(gpu, method):
     # neighbours), and each one is the FP64 answer
     diff, ok = parity.sil2d_differences_explained(poly.snch.silhouette_data(), x2.cpu().numpy(), e, p)
     assert len(diff) < 1e-4 * N_FULL and ok.all(), (len(diff), diff[~ok][:5])
+
+
+@pytest.mark.parametrize("method", ["engine", "engine_fp32"])
+def test_headline_icosphere_s7_cp_full(gpu, method):
+    """The metric's workload: closest point, displaced icosphere s=7 (327,680 triangles), 1,048,576
+    uniform queries.  Every query: distance bit-identical to the GPU brute force over all 327,680
+    triangles (exact pruning), agreement with the plain kernels (the reference's float32 traversal),
+    record invariants; a 16,384-query sample against the oracle and the FP64 brute force."""
+    V, F = synth.displaced_icosphere(7)
+    assert F.shape[0] == 327680
+    shape = Mesh(V, F, device="cuda:0")
+    s = shape.snch
+    x = synth.uniform_points(N_FULL, 3)
+    xt = torch.from_numpy(x).cuda()
+    print(f"[headline] {method}", flush=True)
+    eng = _np(shape.closest_point_bvh(xt, method=method))
+    bf = _np(shape.closest_point_baseline(xt), ("valid", "d", "prim_id"))
+    assert eng["valid"].all()
+    assert np.array_equal(eng["d"].view(np.int32), bf["d"].view(np.int32))
+    pl = _np(shape.closest_point_bvh(xt, method="plain"))
+    assert parity.rel_err(eng["d"], pl["d"]).max() <= parity.TOL_REF
+    # different ids only on ties (a nearest point on a shared vertex or edge: 28% of the queries)
+    tie = eng["prim_id"] != pl["prim_id"]
+    assert (parity.rel_err(eng["d"][tie], pl["d"][tie]) <= 1e-6).all()
+    # record invariants: d = |p - x|, p on its triangle, p = a + u (b - a) + v (c - a) for uv = (u, v)
+    assert parity.rel_err(np.linalg.norm(eng["p"] - x, axis=1), eng["d"]).max() <= 1e-5
+    tri = F[eng["prim_id"]]
+    a, b, c = (V[tri[:, k]].astype(np.float64) for k in range(3))
+    assert _point_triangle_distance(eng["p"], a, b, c).max() <= 1e-5
+    uv = eng["uv"].astype(np.float64)
+    assert (uv >= -1e-6).all() and (uv.sum(1) <= 1 + 1e-6).all()
+    assert np.abs(a + uv[:, :1] * (b - a) + uv[:, 1:] * (c - a) - eng["p"]).max() <= 1e-5
+    nrm = np.cross(b - a, c - a)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    assert np.abs(nrm - eng["n"]).max() <= 1e-5
+    rng = np.random.default_rng(13)
+    sample = rng.choice(N_FULL, 16384, replace=False)
+    ref = O.closest_point(3, s.bvh_node_data(), s.primitive_data(), x[sample])
+    bf64 = O.bf_closest_point(3, V, F, x[sample])
+    got = {k: v[sample] for k, v in eng.items()}
+    res = parity.summarize(f"headline/{method}", got, ref, parity.gap_mask(bf64["d"], bf64["second_d"]))
+    assert res["ok"], parity.brief(res)
+    assert parity.rel_err(got["d"], bf64["d"]).max() <= parity.TOL_F64

@@ -10,6 +10,7 @@ from gquery_amd import ext, synth
 from oracle import oracle as O
 from tests import parity
 from tests.test_builder import PENTAGON_E, PENTAGON_V, TET_F, TET_V
+from tests.test_oracle import TWO_TRI_F, TWO_TRI_HIT, TWO_TRI_RAY, TWO_TRI_V
 
 pytestmark = pytest.mark.gpu
 
@@ -42,7 +43,7 @@ def scene(name):
     return _SCENES[name]
 
 
-def _np(r, fields=("valid", "prim_id", "d", "p", "n", "uv", "on_boundary")):
+def _np(r, fields=("valid", "prim_id", "d", "p", "n", "uv", "t", "on_boundary")):
     import torch
     torch.cuda.synchronize()
     return {f: getattr(r, f).cpu().numpy() for f in fields if hasattr(r, f)}
@@ -93,19 +94,15 @@ def test_parity_vs_oracle(gpu, name, kind, method):
     ref = _oracle(kind, dim, shape.snch, x, v)
     got = _np(_query(shape, kind, method, x, v))
     gap = _gap_from_bruteforce(kind, dim, shape.snch, V, F, x, v, ref)
-    vec = ("p", "n") if kind != "sil" else ("p",)
     keep = np.ones(n, bool)
     if kind == "ray" and method != "plain" and dim == 3:
         # The engine re-evaluates grazing ray-triangle tests in FP64: there it must meet the FP64
         # bound, and the FP32 reference is compared only where it is itself that accurate.
         b = O.bf_intersect(dim, V, F, x, v)
-        hit = np.isfinite(b["t"]) & got["valid"].astype(bool)
-        assert parity.rel_err(got["d"][hit], b["t"][hit]).max(initial=0.0) <= parity.TOL_F64
-        with np.errstate(invalid="ignore"):
-            keep = ~np.isfinite(b["t"]) | (parity.rel_err(ref["d"], b["t"]) <= 0.1 * parity.TOL_REF)
+        keep = parity.ray3d_vs_fp64(got, ref, b, gap)
     sub = lambda r: {k: a[keep] for k, a in r.items() if isinstance(a, np.ndarray) and len(a) == n}
-    res = parity.summarize(f"{name}/{kind}/{method}", sub(got), sub(ref), gap[keep], vec_fields=vec)
-    assert res["ok"], res
+    res = parity.summarize(f"{name}/{kind}/{method}", sub(got), sub(ref), gap[keep])
+    assert res["ok"], parity.brief(res)
     if method == "plain" and kind != "sil":
         # same arithmetic as the oracle: bit-identical records
         assert np.array_equal(got["prim_id"], ref["prim_id"])
@@ -166,9 +163,14 @@ def test_known_answers_2d(gpu, method):
     assert r["valid"][0] and r["prim_id"][0] == 2
     np.testing.assert_allclose(r["p"][0], [1.02, 1.24], atol=1e-6)
     np.testing.assert_allclose(r["d"][0], np.sqrt(0.242), rtol=1e-6)
+    np.testing.assert_allclose(r["t"][0], 0.48, atol=1e-6)                      # segment t
+    np.testing.assert_allclose(r["n"][0], [0.5 / np.sqrt(1.25), 1 / np.sqrt(1.25)], atol=1e-6)
     r = _np(pl.intersect_bvh(np.array([[0.8, 0.8]], np.float32), np.array([[1.0, 1.0]], np.float32), method=method))
     assert r["valid"][0] and r["prim_id"][0] == 2 and r["on_boundary"][0]
     np.testing.assert_allclose(r["d"][0], 0.7 - 1 / 3, rtol=1e-6)
+    np.testing.assert_allclose(r["p"][0], [0.8 + 0.7 - 1 / 3] * 2, atol=1e-6)
+    assert r["t"][0] == -1.0                                                     # bvh.py:267-274
+    np.testing.assert_allclose(r["n"][0], [0.5 / np.sqrt(1.25), 1 / np.sqrt(1.25)], atol=1e-6)
     r = _np(pl.closest_silhouette_snch(torch.tensor([[2.0, 0.5]]), method=method), ("valid", "prim_id", "d", "p"))
     assert r["valid"][0] and r["prim_id"][0] == 2
     np.testing.assert_allclose(r["d"][0], np.sqrt(0.5), rtol=1e-6)
@@ -182,9 +184,19 @@ def test_known_answers_3d(gpu, method):
     assert r["valid"][0] and r["prim_id"][0] == 2
     np.testing.assert_allclose(r["d"][0], 0.2 / np.sqrt(21), rtol=2e-6)
     np.testing.assert_allclose(r["p"][0], np.array([59, 61, 62]) / 105, atol=1e-6)
+    np.testing.assert_allclose(r["uv"][0], [2 / 7, 62 / 105], atol=1e-6)           # (bary_b, bary_c)
+    np.testing.assert_allclose(r["n"][0], np.array([1, 0.5, 0.25]) / np.sqrt(1.3125), atol=1e-6)
     r = _np(m.intersect_bvh(np.array([[0.6, 0.6, 0.6]], np.float32), np.array([[1.0, 1.0, 1.0]], np.float32),
                             method=method))
     assert not r["valid"][0] and r["d"][0] == 0 and r["prim_id"][0] == 0
+    assert not r["uv"].any() and not r["n"].any() and not r["on_boundary"][0]
+    # a hit with hand-derived barycentrics: uv = (1 - v - w, v) = (0.625, 0.25) (triangle.py:48-73)
+    m2 = Mesh(TWO_TRI_V, TWO_TRI_F, device="cuda:0")
+    r = _np(m2.intersect_bvh(*TWO_TRI_RAY, method=method))
+    assert r["valid"][0] and r["on_boundary"][0] and r["prim_id"][0] == TWO_TRI_HIT["prim_id"]
+    np.testing.assert_allclose(r["d"][0], TWO_TRI_HIT["d"], rtol=1e-7)
+    for f in ("p", "n", "uv"):
+        np.testing.assert_allclose(r[f][0], TWO_TRI_HIT[f], atol=1e-7)
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -213,11 +225,11 @@ def test_edge_cases(gpu, method, dim):
     for r_max in (0.3, rm):
         ref = _oracle("ray", dim, s, x, v, r_max)
         got = _np(shape.intersect_bvh(x, v, torch.as_tensor(r_max), method=method))
-        res = parity.summarize("ray r_max", got, ref, None, vec_fields=())
+        res = parity.summarize("ray r_max", got, ref, None)
         assert res["valid_mismatch"] == 0 and res["max_err_d"] < 1e-5, res
         ref = _oracle("sil", dim, s, x, None, r_max)
         got = _np(shape.closest_silhouette_snch(x, torch.as_tensor(r_max), method=method), ("valid", "prim_id", "d", "p"))
-        res = parity.summarize("sil r_max", got, ref, None, vec_fields=())
+        res = parity.summarize("sil r_max", got, ref, None)
         assert res["valid_mismatch"] == 0 and res["max_err_d"] < 1e-5, res
 
 
@@ -265,12 +277,11 @@ def test_wave_per_query_pass(gpu, wave_front, budget, name, kind, method):
         ref = _oracle(kind, dim, shape.snch, x, None, r_max)
         got = _np(_query(shape, kind, method, x, None, r_max))
         gap = _gap_from_bruteforce(kind, dim, shape.snch, V, F, x, None, ref) if r_max == float("inf") else None
-        vec = ("p", "n") if kind == "cp" else ("p",)
-        res = parity.summarize(f"{name}/{kind}/{method}/budget{budget}/r{r_max}", got, ref, gap, vec_fields=vec)
+        res = parity.summarize(f"{name}/{kind}/{method}/budget{budget}/r{r_max}", got, ref, gap)
         if gap is None:   # ids and points only defined up to ties here: valid flags and distances
             assert res["valid_mismatch"] == 0 and res["max_err_d"] < parity.TOL_REF, res
         else:
-            assert res["ok"], res
+            assert res["ok"], parity.brief(res)
 
 
 @pytest.mark.parametrize("traversal", [1, 2])
@@ -287,8 +298,8 @@ def test_silhouette_traversal_modes(gpu, traversal, name, method):
     ref = _oracle("sil", dim, shape.snch, x)
     got = _np(_query(shape, "sil", method, x))
     gap = _gap_from_bruteforce("sil", dim, shape.snch, V, F, x, None, ref)
-    res = parity.summarize(f"{name}/sil/{method}/traversal{traversal}", got, ref, gap, vec_fields=("p",))
-    assert res["ok"], res
+    res = parity.summarize(f"{name}/sil/{method}/traversal{traversal}", got, ref, gap)
+    assert res["ok"], parity.brief(res)
 
 
 @pytest.mark.parametrize("traversal", [1, 2])
@@ -309,8 +320,8 @@ def test_quantized_silhouette_pairs(gpu, traversal, budget, name):
                      structure=base.snch)
         out[pairs] = _np(_query(shape, "sil", "engine", x))
         res = parity.summarize(f"{name}/sil/pairs{pairs}/traversal{traversal}/budget{budget}", out[pairs], ref,
-                               gap, vec_fields=("p",))
-        assert res["ok"], res
+                               gap)
+        assert res["ok"], parity.brief(res)
     assert np.array_equal(out[0]["valid"], out[1]["valid"])
     assert np.array_equal(out[0]["d"], out[1]["d"])
     assert np.array_equal(out[0]["prim_id"][gap], out[1]["prim_id"][gap])
@@ -330,6 +341,5 @@ def test_sorted_batch(gpu, name, kind, method):
     ref = _oracle(kind, dim, shape.snch, x, v)
     got = _np(_query(shape, kind, method, x, v))
     gap = _gap_from_bruteforce(kind, dim, shape.snch, V, F, x, v, ref)
-    vec = ("p", "n") if kind != "sil" else ("p",)
-    res = parity.summarize(f"{name}/{kind}/{method}/sorted", got, ref, gap, vec_fields=vec)
-    assert res["ok"], res
+    res = parity.summarize(f"{name}/{kind}/{method}/sorted", got, ref, gap)
+    assert res["ok"], parity.brief(res)

@@ -25,7 +25,8 @@ def test_results_alone_equal_results_in_the_mix(gpu):
         reqs.append(Request(rid=i, scene=k, kind=kind, x=x, v=v))
     ex = GpuExecutor(shapes, slots=(2, 1))
     sched = Scheduler(ex, memory_budget=256 << 20, slots=(2, 1))
-    arrivals = np.sort(rng.uniform(0.0, 0.05, size=len(reqs)))
+    # a burst (half the requests at once: they must be coalesced), then arrivals spread over 50 ms
+    arrivals = np.concatenate([np.zeros(len(reqs) // 2), np.sort(rng.uniform(0.0, 0.05, size=len(reqs) - len(reqs) // 2))])
     run_open_loop(sched, reqs, arrivals)
     torch.cuda.synchronize()
     assert len(sched.completed) == len(reqs)
@@ -34,7 +35,7 @@ def test_results_alone_equal_results_in_the_mix(gpu):
         alone = (shape.closest_point_bvh(r.x) if r.kind == "cp" else
                  shape.intersect_bvh(r.x, r.v) if r.kind == "ray" else shape.closest_silhouette_snch(r.x))
         torch.cuda.synchronize()
-        for f in ("valid", "p", "d", "prim_id", "n", "uv", "on_boundary"):
+        for f in ("valid", "p", "d", "prim_id", "n", "uv", "t", "on_boundary"):
             if hasattr(alone, f):
                 a, b = getattr(alone, f).cpu().numpy(), getattr(r.result, f).cpu().numpy()
                 assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (r.rid, r.kind, f)

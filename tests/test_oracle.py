@@ -21,7 +21,7 @@ def test_pentagon_closest_point():
     assert r["valid"][0] == 1 and r["prim_id"][0] == 2
     np.testing.assert_allclose(r["p"][0], [1.02, 1.24], atol=1e-6)
     np.testing.assert_allclose(r["d"][0], math.sqrt(0.242), rtol=1e-6)
-    np.testing.assert_allclose(r["uv"][0], 0.48, atol=1e-6)
+    np.testing.assert_allclose(r["t"][0], 0.48, atol=1e-6)
     # n = (t.y, -t.x) of normalize(b - a) = normalize(-1, 0.5)
     np.testing.assert_allclose(r["n"][0], [0.5 / math.sqrt(1.25), 1 / math.sqrt(1.25)], atol=1e-6)
 
@@ -35,7 +35,7 @@ def test_pentagon_ray():
     assert r["valid"][0] == 1 and r["prim_id"][0] == 2 and r["on_boundary"][0] == 1
     np.testing.assert_allclose(r["d"][0], t, rtol=1e-6)
     np.testing.assert_allclose(r["p"][0], [0.8 + t, 0.8 + t], atol=1e-6)
-    assert r["uv"][0] == -1.0
+    assert r["t"][0] == -1.0
 
 
 def test_pentagon_silhouette():
@@ -68,6 +68,60 @@ def test_tetrahedron_ray_misses():
     r = O.intersect(3, s.bvh_node_data(), s.primitive_data(), np.array([[0.6, 0.6, 0.6]], np.float32),
                     np.array([[1.0, 1.0, 1.0]], np.float32))
     assert r["valid"][0] == 0 and r["d"][0] == 0 and r["prim_id"][0] == 0 and not r["p"].any()
+
+
+# Two parallel triangles: a = (0,0,0), b = (2,0,0), c = (0,2,0) and the same 1 below.  The ray from
+# (0.5, 0.25, 1) along (0, 0, -1) meets the upper one first.  Moller-Trumbore (math.py:110-128):
+# v1 = b - a = (2,0,0), v2 = c - a = (0,2,0), p = d x v2 = (2,0,0), det = v1.p = 4, s = x - a,
+# v = s.p / det = 0.25, q = s x v1 = (0, 2, -0.5), w = d.q / det = 0.125, t = v2.q / det = 1.
+# Record (triangle.py:48-73): p = a + v1 v + v2 w = (0.5, 0.25, 0), n = normalize(v1 x v2) = (0,0,1),
+# uv = (1 - v - w, v) = (0.625, 0.25), d = t = 1, on_boundary.
+TWO_TRI_V = np.array([[0, 0, 0], [2, 0, 0], [0, 2, 0], [0, 0, -1], [2, 0, -1], [0, 2, -1]], np.float32)
+TWO_TRI_F = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+TWO_TRI_RAY = (np.array([[0.5, 0.25, 1.0]], np.float32), np.array([[0.0, 0.0, -1.0]], np.float32))
+TWO_TRI_HIT = {"prim_id": 0, "d": 1.0, "p": [0.5, 0.25, 0.0], "n": [0.0, 0.0, 1.0], "uv": [0.625, 0.25]}
+
+
+def test_two_triangle_ray_record():
+    s = ext.SNCH3D(TWO_TRI_V, TWO_TRI_F)
+    r = O.intersect(3, s.bvh_node_data(), s.primitive_data(), *TWO_TRI_RAY)
+    assert r["valid"][0] == 1 and r["on_boundary"][0] == 1 and r["prim_id"][0] == TWO_TRI_HIT["prim_id"]
+    np.testing.assert_allclose(r["d"][0], TWO_TRI_HIT["d"], rtol=1e-7)
+    for f in ("p", "n", "uv"):
+        np.testing.assert_allclose(r[f][0], TWO_TRI_HIT[f], atol=1e-7)
+    # the ray from below meets the lower triangle first: v = 0.25, w = 0.125 again, t = 1
+    r = O.intersect(3, s.bvh_node_data(), s.primitive_data(), np.array([[0.5, 0.25, -2.0]], np.float32),
+                    np.array([[0.0, 0.0, 1.0]], np.float32))
+    assert r["valid"][0] == 1 and r["prim_id"][0] == 1
+    np.testing.assert_allclose(r["p"][0], [0.5, 0.25, -1.0], atol=1e-7)
+    np.testing.assert_allclose(r["uv"][0], [0.625, 0.25], atol=1e-7)
+    # the FP64 brute force reports the same record fields
+    b = O.bf_intersect(3, TWO_TRI_V, TWO_TRI_F, *TWO_TRI_RAY)
+    assert b["prim_id"][0] == 0 and b["t"][0] == 1.0 and b["second_t"][0] == 2.0
+    np.testing.assert_allclose(b["uv"][0], [0.625, 0.25], atol=1e-15)
+
+
+def test_ray_barycentrics_vs_fp64():
+    """The FP32 restatement's ray barycentrics (1 - v - w, v) against the FP64 brute force's, away
+    from grazing angles (|cos| of the hit angle above 0.2) and near ties."""
+    V, F = synth.displaced_icosphere(3, seed=4)
+    s = ext.SNCH3D(V, F)
+    x = synth.uniform_points(6000, 3, seed=13)
+    v = synth.uniform_directions(6000, 3, seed=14)
+    r = O.intersect(3, s.bvh_node_data(), s.primitive_data(), x, v)
+    b = O.bf_intersect(3, V, F, x, v)
+    hit = np.isfinite(b["t"]) & parity.gap_mask(b["t"], b["second_t"]) & (r["prim_id"] == b["prim_id"])
+    tri = V[F[np.where(hit, b["prim_id"], 0)]].astype(np.float64)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    cos = np.abs((nrm * v).sum(1)) / np.linalg.norm(nrm, axis=1)
+    m = hit & (cos > 0.2)
+    assert m.sum() > 1000
+    assert np.abs(r["uv"][m] - b["uv"][m]).max() <= 1e-5
+    # p = a + v (b - a) + w (c - a) of the FP64 barycentrics is the FP64 hit point x + t d
+    u = b["uv"][m]
+    T = tri[m]
+    p = T[:, 0] + u[:, 1:] * (T[:, 1] - T[:, 0]) + (1 - u[:, :1] - u[:, 1:]) * (T[:, 2] - T[:, 0])
+    np.testing.assert_allclose(p, x[m] + b["t"][m][:, None] * v[m], atol=1e-9)
 
 
 def test_tetrahedron_silhouette_matches_brute_force():
