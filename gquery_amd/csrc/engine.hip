@@ -41,7 +41,12 @@ namespace {
 
 using namespace gqd;
 
-constexpr int kBlock = 256;     // packet kernels: 4 wavefronts per block
+constexpr int kBlock = 256;     // sort kernels: 4 wavefronts per block
+// Packet kernels: one wavefront per block.  A block's wave slots and LDS are released only when its
+// slowest wavefront ends, and packet wavefronts differ in length by up to 20x: with 4-wavefront
+// blocks 3D closest point took 7.16 ms on 1M queries, with 1-wavefront blocks 6.93 ms (2-wavefront
+// blocks 7.06 ms); 3D silhouettes 17.2 ms either way.
+constexpr int kPacketBlock = 64;
 constexpr int kRayBlock = 64;   // lane-per-query ray kernel: one wavefront per block
 constexpr int kRayStackMax = 96;  // ray stack entries per lane (768 B of private memory)
 constexpr int kWavePassBlocks = 2048;  // persistent grid of the full-frontier wave-per-query pass
@@ -189,6 +194,25 @@ template <int D> __device__ __forceinline__ float coord_slack(const GP &P, const
 __device__ __forceinline__ float cp_bound2(float d, float slack) {
     const float b = fmaf(d, 1.000001f, slack);
     return b * b;
+}
+
+// Lower bounds on the distance from p to triangle abc that decide whether the exact leaf test can
+// matter: the triangle's own bounding box (squared distance against bound2, the test a one-triangle
+// box would get) and its supporting plane (pl: unit normal, offset; against best, widened for
+// rounding).  Either proves the float32 leaf distance exceeds best.  The plane alone is weak -- a
+// far triangle of a flat patch has its plane near every query -- while the box is tight; together
+// they skip most exact tests of a leaf that only some lanes need.
+__device__ __forceinline__ bool tri_may_beat(const float *pc, const float4 &a, const float4 &b, const float4 &c,
+                                             const float4 &pl, float bound2, float best, float slack) {
+    const float lo0 = fminf(fminf(a.x, b.x), c.x), hi0 = fmaxf(fmaxf(a.x, b.x), c.x);
+    const float lo1 = fminf(fminf(a.y, b.y), c.y), hi1 = fmaxf(fmaxf(a.y, b.y), c.y);
+    const float lo2 = fminf(fminf(a.z, b.z), c.z), hi2 = fmaxf(fmaxf(a.z, b.z), c.z);
+    const float o0 = fmaxf(fmaxf(lo0 - pc[0], pc[0] - hi0), 0.0f);
+    const float o1 = fmaxf(fmaxf(lo1 - pc[1], pc[1] - hi1), 0.0f);
+    const float o2 = fmaxf(fmaxf(lo2 - pc[2], pc[2] - hi2), 0.0f);
+    const float d2 = fmaf(o2, o2, fmaf(o1, o1, o0 * o0));
+    const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
+    return (d2 <= bound2) & (fmaf(lb, 0.99999f, -slack) <= best);
 }
 
 // ---- lane-per-query stacks.  In LDS: depth chosen per launch from the tree depth, one column per
@@ -441,10 +465,86 @@ __device__ __forceinline__ void defer_wave(const GP &P, bool live, uint32_t qi, 
     }
 }
 
+// Number of set bits of a wave mask in the lanes below this one (mbcnt): with one ballot per
+// child slot, the append position of a lane's children is the sum over slots, and the round's
+// total the sum of the masks' popcounts -- no shuffles through LDS.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Packet leaf step for triangles, compacted.  A leaf is opened when any lane needs it, but each of
+// its triangles is typically needed by a quarter of the lanes (the union of 64 search balls is much
+// larger than any one of them), and an exact test run for the whole wavefront wastes the rest.  So
+// the lanes first evaluate the cheap lower bounds of every triangle of a chunk (up to kLeafChunk),
+// append their (triangle, lane) pairs to an LDS list in ballot order, and then the wavefront runs the
+// exact test on 64 pairs at a time: lane t takes pair t, reads the owner's query point from LDS and
+// the triangle from memory, and folds (distance, slot) into the owner's LDS minimum (one 64-bit
+// key: distance bits, which order like unsigned integers for d >= 0, then slot, so ties go to the
+// lowest slot exactly as a sequential scan with that rule).  Each lane then merges its minimum into
+// its answer.  All pairs of a chunk are tested against the bound from before the chunk, which only
+// adds tests, never removes one.
+constexpr int kLeafChunk = 8;
+struct PacketLds {
+    float q[64][3];                 // the lanes' query points
+    unsigned long long mn[64];      // per-lane minimum of a chunk: (distance bits << 32) | slot
+    uint16_t pair[kLeafChunk * 64]; // (triangle in chunk << 6) | lane
+};
+
+__device__ __forceinline__ void cp_leaf_compact(const GP &P, PacketLds &L, uint32_t first, uint32_t count,
+                                                const float *pc, bool live, float slack, float &best,
+                                                int &best_slot, float &bound2) {
+    const int lane = threadIdx.x & 63;
+    for (uint32_t c0 = first; c0 < first + count; c0 += kLeafChunk) {
+        const uint32_t cn = min((uint32_t)kLeafChunk, first + count - c0);
+        int total = 0;
+        for (uint32_t j = 0; j < cn; ++j) {
+            const uint32_t s = c0 + j;
+            const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
+            const float4 pl = ldg(P.prims + (4 * s + 3));
+            const bool need = live && tri_may_beat(pc, a, b, c, pl, bound2, best, slack);
+            const unsigned long long m = __ballot(need);
+            if (need) L.pair[total + lanes_below(m)] = (uint16_t)((j << 6) | (uint32_t)lane);
+            total += __popcll(m);
+        }
+        if (total == 0) continue;
+        L.mn[lane] = ~0ull;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int b0 = 0; b0 < total; b0 += 64) {
+            const int k = b0 + lane;
+            if (k < total) {
+                const uint32_t e = L.pair[k];
+                const uint32_t owner = e & 63u, s = c0 + (e >> 6);
+                const V3 q = v3(L.q[owner][0], L.q[owner][1], L.q[owner][2]);
+                const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
+                V3 pt;
+                V2 uv;
+                const float dd = closest_point_triangle(q, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
+                atomicMin(&L.mn[owner], ((unsigned long long)__float_as_uint(dd) << 32) | s);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const unsigned long long mk = L.mn[lane];
+        if (live && mk != ~0ull) {
+            const float dd = __uint_as_float((uint32_t)(mk >> 32));
+            const int s = (int)(uint32_t)mk;
+            if (dd < best || (dd == best && s < best_slot)) {
+                best = dd;
+                best_slot = s;
+                bound2 = cp_bound2(dd, slack);
+            }
+        }
+    }
+}
+
 template <int D, bool Q>
 __device__ __forceinline__ void cp_packet(const GP &P, int base) {
     using V = typename VT<D>::T;
-    __shared__ uint32_t wstack[kBlock / 64][kPacketDepth];
+    __shared__ uint32_t wstack[kPacketBlock / 64][kPacketDepth];
+    __shared__ PacketLds wlds[D == 3 ? kPacketBlock / 64 : 1];
     uint32_t *wst = wstack[threadIdx.x >> 6];
     const int i = base + threadIdx.x;
     const bool live = i < P.n;
@@ -461,6 +561,10 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
     int best_slot = -1;
     int sp = 0;
     int steps = 0;
+    if constexpr (D == 3) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) wlds[threadIdx.x >> 6].q[threadIdx.x & 63][k] = pc[k];
+    }
     uint32_t cur = uniform(P.root);
     for (;;) {
         if (P.budget > 0 && ++steps > P.budget) {
@@ -524,27 +628,18 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
             leaf_range(cur, (const uint32_t *)P.big, first, count);
             first = uniform(first);
             count = uniform(count);
-            for (uint32_t s = first; s < first + count; ++s) {
-                float dd;
-                if constexpr (D == 3) {
-                    const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
-                    const float4 pl = ldg(P.prims + (4 * s + 3));
-                    // plane-distance lower bound, widened for rounding (relative and absolute):
-                    // skip the exact test when no lane can get closer through this triangle
-                    const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
-                    if (__ballot(fmaf(lb, 0.99999f, -slack) <= best) == 0ull) continue;
-                    V3 pt;
-                    V2 uv;
-                    dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
-                } else {
+            if constexpr (D == 3) {
+                cp_leaf_compact(P, wlds[threadIdx.x >> 6], first, count, pc, live, slack, best, best_slot, bound2);
+            } else {
+                for (uint32_t s = first; s < first + count; ++s) {
                     const float4 ab = ldg(P.prims + (s));
                     V2 pt;
-                    dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
-                }
-                if (live && (dd < best || (dd == best && (int)s < best_slot))) {
-                    best = dd;
-                    best_slot = (int)s;
-                    bound2 = cp_bound2(dd, slack);
+                    const float dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
+                    if (live && (dd < best || (dd == best && (int)s < best_slot))) {
+                        best = dd;
+                        best_slot = (int)s;
+                        bound2 = cp_bound2(dd, slack);
+                    }
                 }
             }
         }
@@ -618,12 +713,6 @@ __device__ __forceinline__ float wave_min(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// Number of set bits of a wave mask in the lanes below this one (mbcnt): with one ballot per
-// child slot, the append position of a lane's children is the sum over slots, and the round's
-// total the sum of the masks' popcounts -- no shuffles through LDS.
-__device__ __forceinline__ int lanes_below(unsigned long long m) {
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
 
 // Appends one query (index, bound, best slot, job) to a hand-off list: every lane takes part in the
 // atomic (lane 0 adds 1) and stores the same entry, so no lane-dependent branch is involved.
@@ -680,10 +769,9 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                     for (uint32_t s = first; s < first + count; ++s) {
                         float dd;
                         if constexpr (D == 3) {
-                            const float4 pl = ldg(P.prims + (4 * s + 3));
-                            const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
-                            if (fmaf(lb, 0.99999f, -slack) > best) continue;
                             const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
+                            const float4 pl = ldg(P.prims + (4 * s + 3));
+                            if (!tri_may_beat(pc, a, b, c, pl, bound2, best, slack)) continue;
                             V3 pt;
                             V2 uv;
                             dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
@@ -1075,8 +1163,8 @@ __device__ __forceinline__ bool ref_cone_pass(const GP &P, const PairChild &c, c
 template <int D, bool Q>
 __device__ __forceinline__ void sil_packet(const GP &P, int base) {
     using V = typename VT<D>::T;
-    __shared__ uint32_t wstack[kBlock / 64][kPacketDepth];
-    __shared__ unsigned long long wmask[kBlock / 64][kPacketDepth];
+    __shared__ uint32_t wstack[kPacketBlock / 64][kPacketDepth];
+    __shared__ unsigned long long wmask[kPacketBlock / 64][kPacketDepth];
     uint32_t *wst = wstack[threadIdx.x >> 6];
     unsigned long long *wmk = wmask[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
@@ -1689,15 +1777,15 @@ __global__ void __launch_bounds__(kBlock) k_sort_refine(const EngineParams *J) {
 // tiles; the wave-per-query passes drain the deferred list of their launch group, whose entries
 // name their job.
 template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_cp_packet(const EngineParams *J, const uint2 *tiles, int spread) {
-    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kBlock);
+__global__ void __launch_bounds__(kPacketBlock) k_cp_packet(const EngineParams *J, const uint2 *tiles, int spread) {
+    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kPacketBlock);
     const GP P = gview(J[tl.x]);
     cp_packet<D, Q>(P, (int)tl.y);
 }
 
 template <int D, bool Q>
-__global__ void __launch_bounds__(kBlock) k_sil_packet(const EngineParams *J, const uint2 *tiles, int spread) {
-    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kBlock);
+__global__ void __launch_bounds__(kPacketBlock) k_sil_packet(const EngineParams *J, const uint2 *tiles, int spread) {
+    const uint2 tl = tile_at(tiles, xcd_tile(blockIdx.x, gridDim.x, spread), kPacketBlock);
     const GP P = gview(J[tl.x]);
     sil_packet<D, Q>(P, (int)tl.y);
 }
@@ -1834,7 +1922,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
         plan.n_sort_tiles[di] = plan.tiles.size() - plan.sort_tiles_off[di];
     }
     for (Group &G : plan.groups) {
-        const int ts = G.lane ? kRayBlock : kBlock;
+        const int ts = G.lane ? kRayBlock : kPacketBlock;
         G.tiles_off = plan.tiles.size();
         for (int j : G.jobs)
             for (int64_t t = 0; t < jobs[j].q.n; t += ts) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
@@ -1999,11 +2087,11 @@ void launch_group_t(const Group &G, const EngineParams *J, const uint2 *tiles, u
         hipLaunchKernelGGL((k_sil_lane<D, Q>), dim3(nt), dim3(kRayBlock), lds, st, J, gt, xcd_spread());
     } else if (G.kind == kSilhouette) {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 4);
-        hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt, xcd_spread());
+        hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kPacketBlock), 0, st, J, gt, xcd_spread());
         if (dc) launch_wave_pass(k_sil_wave<D, Q>, G, J, dc, def, ovf, st);
     } else {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 4);
-        hipLaunchKernelGGL((k_cp_packet<D, Q>), dim3(nt), dim3(kBlock), 0, st, J, gt, xcd_spread());
+        hipLaunchKernelGGL((k_cp_packet<D, Q>), dim3(nt), dim3(kPacketBlock), 0, st, J, gt, xcd_spread());
         if (dc) launch_wave_pass(k_cp_wave<D, Q>, G, J, dc, def, ovf, st);
     }
 }
