@@ -5,8 +5,16 @@ triangles), 1,048,576 query points uniform in [-1.5, 1.5]^3, all resident in HBM
 A step is one Mesh.closest_point_bvh call over the whole batch.  The CPU baseline is the FP32
 restatement of the reference traversal (oracle/gq_oracle.c) on the host cores, over a bounded
 sample of the same queries.  The same run times the plain FP32 kernels on the same batch
-(`vs_plain`) and the serving loop (16 resident scenes, Poisson requests at 25/50/90% of the
-saturated rate: p50/p99 latency under `served`).
+(`vs_plain`), every other query path on its configuration workload (`paths`: 3D rays and
+silhouettes on the same mesh, 2D closest point / rays / silhouettes on the 65,536-segment star,
+closest point unsorted and below the sort threshold), the uneven batched workload (`batched`: 64
+scenes, one graph replay per step) and the serving loop (16 resident scenes, Poisson requests at
+25/50/90% of the saturated rate: p50/p99 latency under `served`).
+
+The roofline names the dominant kernel's bound: the closest-point packet kernel is limited by VALU
+issue, so `roofline` is its VALU instruction rate (instructions per launch from the committed
+rocprofv3 SQ counter summary, kernel time measured in this run) against the issue peak, with the
+HBM traffic of the traversal kernels (counter summary) against 8 TB/s beside it.
 
     python bench.py [--steps K] [--warmup W] [--method engine|engine_fp32|plain]
 """
@@ -23,6 +31,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of MI355X in wave64 instructions per second.  The published FP32 vector peak,
+# 157.3 TFLOP/s, counts packed FMAs: 256 CUs x 4 SIMDs x 2.4 GHz x 16 lanes x 2 (FMA) x 2 (packed)
+# per clock.  A wave64 VALU instruction therefore takes a SIMD 4 clocks: 1,024 SIMDs x 2.4e9 / 4 =
+# 614.4e9 instructions per second, the bound of kernels that are limited by VALU issue.
+VALU_PEAK_GIPS = 157.3e12 / (64 * 2 * 2) / 1e9
+PATH_STEPS = 10
 PLAIN_STEPS = 3
 N_QUERIES = 1 << 20
 SUBDIV = 7
@@ -121,6 +135,108 @@ def pmc_summary(args):
         res["l2_hit_rate"] = hits / (hits + misses) if hits + misses else None
     (ROOT / "profiles" / f"{tag}.json").write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res, indent=1))
+
+
+def _engine_sha():
+    import hashlib
+    return hashlib.sha256((ROOT / "gquery_amd" / "csrc" / "engine.hip").read_bytes()).hexdigest()[:16]
+
+
+def sq_summary(args):
+    """No GPU: SQ counters of rocprofv3 --pmc runs (csv; each its own pass) of a bench command into
+    profiles/counters_<tag>.json: mean per dispatch for every traversal kernel, with the hash of the
+    engine source the run was built from (bench.py reports whether it still matches)."""
+    merged = {}
+    for d in args.sq_summary:
+        for k, c in _pmc_table(d).items():
+            if "gqa::" not in k or not any(t in k for t in ("k_cp_", "k_ray_", "k_sil_")):
+                continue
+            merged.setdefault(k[:160], {}).update(c)
+    res = {"engine_src_sha": _engine_sha(), "tag": args.sq_tag, "kernels": merged}
+    (ROOT / "profiles" / f"counters_{args.sq_tag}.json").write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res, indent=1))
+
+
+def valu_roofline(method, packet_ms, traversal_ms, kernel_name):
+    """VALU-issue roofline of the packet kernel from the committed SQ counter summary (instructions
+    per launch) and the kernel time measured in this run, with the HBM traffic of all traversal
+    kernels (FETCH_SIZE + WRITE_SIZE summary) against the HBM peak beside it."""
+    out = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_GIPS, "unit": "G wave64 VALU instr/s",
+           "frac": None, "traffic": None, "kernel": kernel_name, "kernel_ms": packet_ms,
+           "traversal_ms": traversal_ms}
+    cf = ROOT / "profiles" / f"counters_cp3d_s7_1M_{method}.json"
+    if cf.exists():
+        c = json.loads(cf.read_text())
+        pk = [k for k in c["kernels"] if "k_cp_packet<3" in k]
+        if pk and packet_ms:
+            valu = c["kernels"][pk[0]].get("SQ_INSTS_VALU")
+            out["valu_instr_per_launch"] = valu
+            out["achieved"] = valu / (packet_ms * 1e-3) / 1e9
+            out["frac"] = out["achieved"] / VALU_PEAK_GIPS
+            out["counters"] = f"profiles/{cf.name}"
+            out["counters_match_build"] = c.get("engine_src_sha") == _engine_sha()
+    tf = ROOT / "profiles" / f"traffic_cp3d_s7_1M_{method}.json"
+    if tf.exists():
+        t = json.loads(tf.read_text())
+        out["traffic"] = t.get("hbm_bytes_per_launch")
+        if out["traffic"] and traversal_ms:
+            gbs = out["traffic"] / (traversal_ms * 1e-3) / 1e9
+            out["hbm"] = {"achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
+                          "l2_hit_rate": t.get("l2_hit_rate"), "summary": f"profiles/{tf.name}"}
+    return out
+
+
+def time_call(fn, steps=PATH_STEPS, warmup=2):
+    """(ms per step from events around the steps, traversal ms and first-kernel ms per launch from
+    the library's own events on the launch stream)."""
+    import torch
+    from gquery_amd import _lib
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    _lib.profile_reset()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    tot, first, n, name = _lib.profile_read_split()
+    return e0.elapsed_time(e1) / steps, tot / max(n, 1), first / max(n, 1), name
+
+
+def query_paths(mesh, poly, x3, v3, x2, v2, log):
+    """Every query path on its configuration workload: 3D on the s=7 icosphere and 2D on the
+    65,536-segment star, 1,048,576 queries each; plus the closest-point query unsorted (caller's
+    order, incoherent packets) and below the sort threshold."""
+    from gquery_amd.batch import Batch
+    from gquery_amd.shapes.mesh import Mesh
+    nq = x3.shape[0]
+    mesh_q8 = Mesh(None, None, device="cuda:0", structure=mesh.snch, sil_pairs=1)
+    cases = {
+        "cp3_engine_fp32": lambda: mesh.closest_point_bvh(x3, method="engine_fp32"),
+        "ray3": lambda: mesh.intersect_bvh(x3, v3),
+        "sil3": lambda: mesh.closest_silhouette_snch(x3),
+        "sil3_pairs_q8": lambda: mesh_q8.closest_silhouette_snch(x3),
+        "cp2": lambda: poly.closest_point_bvh(x2),
+        "ray2": lambda: poly.intersect_bvh(x2, v2),
+        "sil2": lambda: poly.closest_silhouette_snch(x2),
+        "sil2_engine_fp32": lambda: poly.closest_silhouette_snch(x2, method="engine_fp32"),
+    }
+    b_unsorted = Batch([{"shape": mesh, "kind": "cp", "x": x3, "order": 1}])
+    cases["cp3_input_order"] = b_unsorted.run
+    small = x3[:8000]
+    cases["cp3_8000_queries"] = lambda: mesh.closest_point_bvh(small)
+    out = {}
+    for name, fn in cases.items():
+        ms, trav, first, kern = time_call(fn)
+        n = 8000 if name == "cp3_8000_queries" else nq
+        out[name] = {"ms_per_step": ms, "Mq_per_s": n / (ms * 1e-3) / 1e6, "traversal_ms": trav,
+                     "first_kernel_ms": first, "queries": n}
+        log(f"path {name}: {ms:.3f} ms/step, {out[name]['Mq_per_s']:.1f} Mq/s (traversal {trav:.3f} ms, "
+            f"first kernel {first:.3f} ms)")
+    return out
 
 
 def accuracy(args):
@@ -286,7 +402,7 @@ def batched(args):
            "sequential_single_calls_ms": ms_seq, "graph_speedup_vs_sequential": ms_seq / ms_graph,
            "device_workspace_bytes": b.device_bytes,
            "autotuned_options": None if tuned is None else sorted({json.dumps(o, sort_keys=True) for o in tuned})}
-    print(json.dumps(out))
+    return out
 
 
 def served(args, loads=(0.25, 0.5, 0.9)):
@@ -410,6 +526,11 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after timing, run 3 elementwise float copies of this many MiB (the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE counter runs)")
+    ap.add_argument("--sq-summary", nargs="+", metavar="DIR",
+                    help="no GPU: rocprofv3 --pmc csv runs with SQ counters into profiles/counters_<--sq-tag>.json")
+    ap.add_argument("--sq-tag", default="cp3d_s7_1M_engine")
+    ap.add_argument("--no-paths", action="store_true", help="skip timing the other query paths")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-workload leg")
     ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
                     help="no GPU: turn two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; csv) of "
                          "`bench.py --copy-mb M`, and optionally a third (TCC_HIT_sum TCC_MISS_sum), "
@@ -437,7 +558,7 @@ def main():
         print(json.dumps(served(args)))
         return
     if args.batched:
-        batched(args)
+        print(json.dumps(batched(args)))
         return
     if args.tune:
         import torch
@@ -457,6 +578,9 @@ def main():
         return
     if args.pmc_summary:
         pmc_summary(args)
+        return
+    if args.sq_summary:
+        sq_summary(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; queries shard by batch with no "
@@ -525,7 +649,7 @@ def main():
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    kern_ms, kern_launches, kern_name = _lib.profile_read()
+    kern_ms, first_ms, kern_launches, kern_name = _lib.profile_read_split()
     ms_per_step = ms_total / args.steps
     value = nq / (ms_per_step * 1e-3) / 1e6
     log(f"{method} {args.shape} {args.query} s={args.subdiv} n={nq} leaf={args.leaf_prims} budget={args.packet_budget}: {ms_per_step:.3f} ms/step ({wall / args.steps * 1e3:.3f} ms wall), {value:.1f} Mq/s; "
@@ -551,22 +675,20 @@ def main():
         print(json.dumps(line))
         return
 
-    # roofline of the dominant (traversal) kernel
+    # roofline of the dominant kernel (the packet pass, k_cp_packet): VALU issue, with HBM beside it
     rng = np.random.default_rng(0)
+    trav_ms = kern_ms / max(kern_launches, 1)
+    packet_ms = first_ms / max(kern_launches, 1)
+    roof = valu_roofline(method, packet_ms, trav_ms,
+                         f"k_cp_packet<3, {'true' if method == 'engine' else 'false'}> ({kern_name})")
+    # informational: the reference-layout bytes the reference traversal would move (B_q per query,
+    # work counts from the CPU restatement on 20,000 sampled queries) per second of traversal
     sample = x_host[rng.choice(N_QUERIES, 20000, replace=False)]
     bq, counts = algorithmic_bytes_per_query(mesh, sample)
-    kern_avg_s = kern_ms / max(kern_launches, 1) * 1e-3
-    # inputs + records + the reference-layout scene read once
-    compulsory = (S_IN + S_OUT) * N_QUERIES + 4 * (mesh.snch.bvh_node_data().size + mesh.snch.primitive_data().size)
-    achieved = bq * N_QUERIES / kern_avg_s / 1e9 if kern_avg_s > 0 else None
-    traffic = None
-    tf = ROOT / "profiles" / f"traffic_cp3d_s7_1M_{method}.json"
-    if tf.exists():
-        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-            "kernel": kern_name, "kernel_ms": kern_avg_s * 1e3, "bytes_per_query": bq, "work_per_query": counts,
-            "compulsory_bytes_per_launch": compulsory}
+    roof["reference_layout"] = {"bytes_per_query": bq, "work_per_query": counts,
+                                "rate_GBs": bq * N_QUERIES / (trav_ms * 1e-3) / 1e9 if trav_ms else None}
+    log(f"roofline: packet kernel {packet_ms:.3f} ms of {trav_ms:.3f} ms traversal; VALU frac "
+        f"{roof['frac']}; HBM {roof.get('hbm')}")
 
     # sanity: a sample of the timed results against the oracle
     from oracle import oracle as O
@@ -619,6 +741,27 @@ def main():
                  "speedup": p_ms / ms_per_step}
         log(f"plain FP32 kernels: {p_ms:.2f} ms/step, {plain['value']:.1f} Mq/s; engine speedup {plain['speedup']:.2f}x")
 
+    paths = None
+    if not args.no_paths:
+        from gquery_amd.shapes.polyline import Polyline
+        V2, F2 = synth.star_polyline(65536)
+        poly = Polyline(V2, F2, device="cuda:0")
+        x2 = torch.from_numpy(synth.uniform_points(N_QUERIES, 2)).cuda()
+        v2 = torch.from_numpy(synth.uniform_directions(N_QUERIES, 2)).cuda()
+        paths = query_paths(mesh, poly, x, v, x2, v2, log)
+        del poly, x2, v2
+    bat = None
+    if not args.no_batched:
+        bsteps, bwarm = args.steps, args.warmup
+        args.steps, args.warmup = 5, 1
+        b = batched(args)
+        args.steps, args.warmup = bsteps, bwarm
+        bat = {"ms_per_step": b["ms_per_step"], "Mq_per_s": b["value"], "workload": b["config"]["workload"],
+               "sequential_single_calls_ms": b["sequential_single_calls_ms"],
+               "graph_speedup_vs_sequential": b["graph_speedup_vs_sequential"],
+               "device_workspace_bytes": b["device_workspace_bytes"], "steps": 5}
+        torch.cuda.empty_cache()
+
     srv = None
     if not args.no_served:
         del rec
@@ -647,6 +790,8 @@ def main():
         "vs_plain": plain["speedup"] if plain else None,
         "plain": plain,
         "host_memory": host_rate,
+        "paths": paths,
+        "batched": bat,
         "served": srv,
         "max_rel_err_vs_oracle": max_err,
     }

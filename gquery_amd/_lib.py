@@ -63,7 +63,8 @@ class Job(C.Structure):
         ("n", C.c_int64),
         ("out", Records),
         ("traversal", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("order", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -108,6 +109,7 @@ SIGNATURES = {
     "gq_structure_geometry": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.c_void_p, _i64p]),
     "gq_profile_begin": (C.c_int, []),
     "gq_profile_end": (C.c_int, [C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
+    "gq_profile_end_split": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
 }
 
 _lib = None
@@ -151,8 +153,16 @@ def profile_reset() -> None:
 
 def profile_read():
     """-> (total traversal-kernel ms, launches, kernel name) since profile_reset()."""
+    ms, _, n, name = profile_read_split()
+    return ms, n, name
+
+
+def profile_read_split():
+    """-> (total traversal ms, first-kernel ms, launches, kernel name) since profile_reset(); the
+    first kernel is the packet (or lane) kernel of each launch, the wave passes follow it."""
     ms = C.c_double(0)
+    first = C.c_double(0)
     n = C.c_int64(0)
     buf = C.create_string_buffer(128)
-    check(lib().gq_profile_end(C.byref(ms), C.byref(n), buf, 128), "gq_profile_end")
-    return ms.value, n.value, buf.value.decode()
+    check(lib().gq_profile_end_split(C.byref(ms), C.byref(first), C.byref(n), buf, 128), "gq_profile_end_split")
+    return ms.value, first.value, n.value, buf.value.decode()

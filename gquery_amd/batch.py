@@ -40,7 +40,8 @@ class Batch:
     """A fixed set of jobs (shape or Scene, kind, x[, v][, r_max]) with options per job.
 
     kind is "cp", "ray" or "sil"; method "engine" (quantized nodes) or "engine_fp32";
-    packet_budget / traversal override the scene's settings for that job (0 = scene's)."""
+    packet_budget / traversal override the scene's settings for that job (0 = scene's); order=1
+    traverses the job in the caller's order (no Morton sort)."""
 
     def __init__(self, jobs, method: str = "engine", device=None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -75,6 +76,7 @@ class Batch:
             jb.n = n
             jb.out = r
             jb.traversal = int(j.get("traversal", 0))
+            jb.order = int(j.get("order", 0))
         self._jobs = arr
         self._n = len(jobs)
         dev, host = C.c_int64(), C.c_int64()

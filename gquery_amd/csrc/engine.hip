@@ -60,6 +60,25 @@ constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its o
 // 750 vs 3.85 at 2,000 and 8.93 at 128: a 2D query has too little work for a whole wavefront).
 // A scene or job option overrides it.
 constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil3 = 128, kPacketBudgetSil2 = 750;
+// A launch with few queries leaves most of the GPU idle, and then a packet wavefront's walk -- up to
+// its budget of dependent node fetches -- is the latency of the whole call, while the wave-per-query
+// pass puts 64 lanes on each query.  So below a launch size the default budget drops: measured on
+// the 327,680-triangle icosphere (3D closest point, ms per call, default budget vs 64): 300
+// queries 1.80 vs 0.48, 16,384 2.29 vs 0.76, 65,536 3.09 vs 1.45, 262,144 3.95 vs 3.90, 1,048,576
+// 7.04 vs 13.4; 65,536-segment star (2D closest point): 16,384 1.06 vs 0.45, 65,536 1.20 vs 1.03,
+// 262,144 1.47 vs 3.35; 3D silhouettes at budget 1: 300 0.42 vs 0.25, 65,536 1.53 vs 1.36.
+constexpr int kSmallBudgetCp = 64, kSmallBudgetSil3 = 1;
+constexpr int64_t kSmallLaunchCp3 = 1 << 19, kSmallLaunchCp2 = 1 << 17, kSmallLaunchSil3 = 1 << 18;
+
+// Default packet budget of a launch group of `group_q` queries (no job or scene override).
+int default_budget(int kind, int dim, int64_t group_q) {
+    if (kind == kSilhouette) {
+        if (dim == 3) return group_q < kSmallLaunchSil3 ? kSmallBudgetSil3 : kPacketBudgetSil3;
+        return kPacketBudgetSil2;
+    }
+    if (dim == 3) return group_q < kSmallLaunchCp3 ? kSmallBudgetCp : kPacketBudgetCp3;
+    return group_q < kSmallLaunchCp2 ? kSmallBudgetCp : kPacketBudgetCp2;
+}
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
 // Absolute rounding slack per unit of coordinate magnitude: 32 float32 ulps of 1.  The float32
@@ -1882,6 +1901,7 @@ int cell_bits_for(int64_t n, int dim) {
 // fast in input order (3D 0.70 vs 0.69 ms, 2D 0.29 vs 0.33 ms) while the sort costs 0.2 ms.
 // Closest-point and silhouette queries gain 1.2x (3D silhouettes) to 18x (2D closest point).
 bool sort_job(int kind, int64_t n) { return n >= kSortMin && kind != kIntersect; }
+bool sort_job(const EngineJob &jb) { return !jb.input_order && sort_job(jb.kind, jb.q.n); }
 
 // Fills the plan for `jobs` with every device pointer relative to `base` (null when only the
 // sizes are wanted).
@@ -1916,7 +1936,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
     for (int di = 0; di < 2; ++di) {
         plan.sort_tiles_off[di] = plan.tiles.size();
         for (size_t j = 0; j < nj; ++j) {
-            if (jobs[j].w->dim != di + 2 || !sort_job(jobs[j].kind, jobs[j].q.n)) continue;
+            if (jobs[j].w->dim != di + 2 || !sort_job(jobs[j])) continue;
             for (int64_t t = 0; t < jobs[j].q.n; t += kBlock) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
         }
         plan.n_sort_tiles[di] = plan.tiles.size() - plan.sort_tiles_off[di];
@@ -1965,7 +1985,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             P.ids = w.prim_ids;
             P.root = w.root_ref;
         }
-        if (sort_job(jb.kind, jb.q.n)) {
+        if (sort_job(jb)) {
             P.cell_bits = cell_bits_for(jb.q.n, w.dim);
             P.n_cells = 1 << (P.cell_bits * w.dim);
             P.cell_shift = w.dim * ((w.dim == 3 ? 10 : 16) - P.cell_bits);
@@ -1979,6 +1999,8 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
     }
     for (Group &G : plan.groups) {
         size_t nq = 0;
+        int64_t group_q = 0;
+        for (int j : G.jobs) group_q += jobs[j].q.n;
         for (int j : G.jobs) {
             EngineParams &P = plan.P[j];
             const WideView &w = *jobs[j].w;
@@ -1993,9 +2015,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             const bool fits = need <= kFrontMax;
             const int budget = jobs[j].budget != 0 ? jobs[j].budget : w.packet_budget;
             if (fits && budget >= 0) {
-                P.budget = budget > 0 ? budget
-                         : G.kind == kSilhouette ? (G.dim == 3 ? kPacketBudgetSil3 : kPacketBudgetSil2)
-                         : G.dim == 3 ? kPacketBudgetCp3 : kPacketBudgetCp2;
+                P.budget = budget > 0 ? budget : default_budget((int)G.kind, G.dim, group_q);
                 G.defer = true;
                 G.front = std::max(G.front, need);
                 const int small = w.wave_front > 0 ? w.wave_front
@@ -2082,16 +2102,20 @@ void launch_group_t(const Group &G, const EngineParams *J, const uint2 *tiles, u
     uint4 *ovf = G.defer && G.front > G.front_small ? (uint4 *)(base + G.ovf_off) : nullptr;
     if (G.kind == kIntersect) {
         hipLaunchKernelGGL((k_ray_lane<D, Q>), dim3(nt), dim3(kRayBlock), 0, st, J, gt, xcd_spread());
+        prof_mark_first(st);
     } else if (G.kind == kSilhouette && G.lane) {
         const size_t lds = D == 3 ? 0 : (size_t)G.max_stack * kRayBlock * sizeof(uint2);
         hipLaunchKernelGGL((k_sil_lane<D, Q>), dim3(nt), dim3(kRayBlock), lds, st, J, gt, xcd_spread());
+        prof_mark_first(st);
     } else if (G.kind == kSilhouette) {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 4);
         hipLaunchKernelGGL((k_sil_packet<D, Q>), dim3(nt), dim3(kPacketBlock), 0, st, J, gt, xcd_spread());
+        prof_mark_first(st);
         if (dc) launch_wave_pass(k_sil_wave<D, Q>, G, J, dc, def, ovf, st);
     } else {
         if (dc) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, dc, 4);
         hipLaunchKernelGGL((k_cp_packet<D, Q>), dim3(nt), dim3(kPacketBlock), 0, st, J, gt, xcd_spread());
+        prof_mark_first(st);
         if (dc) launch_wave_pass(k_cp_wave<D, Q>, G, J, dc, def, ovf, st);
     }
 }

@@ -45,9 +45,12 @@ namespace {
 struct Prof {
     std::mutex mu;
     bool on = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    // per traversal launch: events before it, after its first kernel (null when none was marked)
+    // and after it
+    struct Rec { hipEvent_t begin, first, end; };
+    std::vector<Rec> ev;
     std::string name;
-    hipEvent_t pending = nullptr;
+    hipEvent_t pending = nullptr, pending_first = nullptr;
 };
 Prof &prof() {
     static Prof p;
@@ -63,6 +66,14 @@ void prof_begin(hipStream_t st) {
     hipEventRecord(p.pending, st);
 }
 
+void prof_mark_first(hipStream_t st) {
+    Prof &p = prof();
+    std::lock_guard<std::mutex> g(p.mu);
+    if (!p.on || !p.pending || p.pending_first) return;
+    if (hipEventCreate(&p.pending_first) != hipSuccess) { p.pending_first = nullptr; return; }
+    hipEventRecord(p.pending_first, st);
+}
+
 void prof_end(hipStream_t st, const char *kernel) {
     Prof &p = prof();
     std::lock_guard<std::mutex> g(p.mu);
@@ -70,8 +81,9 @@ void prof_end(hipStream_t st, const char *kernel) {
     hipEvent_t e1;
     if (hipEventCreate(&e1) != hipSuccess) return;
     hipEventRecord(e1, st);
-    p.ev.emplace_back(p.pending, e1);
+    p.ev.push_back({p.pending, p.pending_first, e1});
     p.pending = nullptr;
+    p.pending_first = nullptr;
     p.name = kernel;
 }
 
@@ -423,12 +435,13 @@ static int batch_jobs(int n_jobs, const gq_job *jobs, std::vector<EngineJob> &ou
         if (kind == kSilhouette && !b.scene->has_snch) return fail(GQ_ERR_UNSUPPORTED, at + "silhouettes need an SNCH scene");
         if (!engine_supported(kind, b.scene->wide)) return fail(GQ_ERR_UNSUPPORTED, at + "tree deeper than the engine's stack");
         if (b.traversal < 0 || b.traversal > 2) return fail(GQ_ERR_INVALID, at + "traversal must be 0, 1 or 2");
+        if (b.order < 0 || b.order > 1) return fail(GQ_ERR_INVALID, at + "order must be 0 or 1");
         if (device >= 0 && b.scene->device != device) return fail(GQ_ERR_INVALID, at + "scenes on different devices");
         device = b.scene->device;
         if (b.n == 0) continue;
         QueryArgs q{b.x, b.v, b.r_max, b.r_max_scalar, b.n, b.out};
         out.push_back(EngineJob{&b.scene->wide, kind, quantized_layout(b.method, kind, b.scene->wide), q, b.packet_budget,
-                                b.traversal != 0 ? b.traversal : b.scene->wide.traversal});
+                                b.traversal != 0 ? b.traversal : b.scene->wide.traversal, b.order == 1});
     }
     return GQ_OK;
 }
@@ -494,31 +507,42 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     return GQ_OK;
 }
 
+static void prof_destroy(Prof::Rec &e) {
+    hipEventDestroy(e.begin);
+    if (e.first) hipEventDestroy(e.first);
+    hipEventDestroy(e.end);
+}
+
 int gq_profile_begin(void) {
     Prof &p = prof();
     std::lock_guard<std::mutex> g(p.mu);
-    for (auto &e : p.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    for (auto &e : p.ev) prof_destroy(e);
     p.ev.clear();
     p.name.clear();
     p.on = true;
     return GQ_OK;
 }
 
-int gq_profile_end(double *total_ms, int64_t *launches, char *name, int32_t name_len) {
+int gq_profile_end_split(double *total_ms, double *first_kernel_ms, int64_t *launches, char *name,
+                         int32_t name_len) {
     Prof &p = prof();
     std::lock_guard<std::mutex> g(p.mu);
     p.on = false;
-    double tot = 0.0;
+    double tot = 0.0, first = 0.0;
     for (auto &e : p.ev) {
-        hipError_t r = hipEventSynchronize(e.second);
+        hipError_t r = hipEventSynchronize(e.end);
         if (r != hipSuccess) return hip_fail(r, "hipEventSynchronize");
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, e.first, e.second);
+        hipEventElapsedTime(&ms, e.begin, e.end);
         tot += ms;
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        if (e.first) {
+            hipEventElapsedTime(&ms, e.begin, e.first);
+            first += ms;
+        }
+        prof_destroy(e);
     }
     if (total_ms) *total_ms = tot;
+    if (first_kernel_ms) *first_kernel_ms = first;
     if (launches) *launches = (int64_t)p.ev.size();
     if (name && name_len > 0) {
         std::strncpy(name, p.name.c_str(), (size_t)name_len - 1);
@@ -526,6 +550,10 @@ int gq_profile_end(double *total_ms, int64_t *launches, char *name, int32_t name
     }
     p.ev.clear();
     return GQ_OK;
+}
+
+int gq_profile_end(double *total_ms, int64_t *launches, char *name, int32_t name_len) {
+    return gq_profile_end_split(total_ms, nullptr, launches, name, name_len);
 }
 
 // ---- Brute-force queries and normals (Polyline / Mesh *_baseline, configure_normal,

@@ -98,5 +98,7 @@ void set_error(const std::string &msg);
 // Traversal-kernel timing (gq_profile_begin / gq_profile_end); no-ops unless enabled.
 void prof_begin(hipStream_t st);
 void prof_end(hipStream_t st, const char *kernel);
+// Marks the end of the first traversal kernel of the launch being profiled.
+void prof_mark_first(hipStream_t st);
 
 } // namespace gqa

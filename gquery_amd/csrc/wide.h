@@ -189,6 +189,7 @@ struct EngineJob {
     QueryArgs q;
     int budget;
     int traversal;
+    bool input_order = false;   // traverse in the caller's order (no Morton sort)
 };
 
 // One batch on `st`: scratch comes from the stream-ordered allocator and is released after use.

@@ -194,7 +194,10 @@ typedef struct gq_job {
     int64_t n;
     gq_records out;          /* as for the single-batch calls of the same kind                  */
     int32_t traversal;       /* 0 = the scene's setting                                         */
-    int32_t reserved[3];
+    int32_t order;           /* 0 = Morton-sort closest-point and silhouette batches of 8,192 or
+                                more queries (the default); 1 = traverse in the caller's order
+                                (queries already coherent, or to measure the unsorted case)     */
+    int32_t reserved[2];
 } gq_job;
 
 /* Bytes of device workspace and of pinned host staging that gq_batch_enqueue needs for `jobs`. */
@@ -263,6 +266,12 @@ int gq_profile_begin(void);
 /* Stops recording, waits for the recorded events and returns the summed traversal-kernel time,
  * the number of launches and the kernel's name (truncated to name_len bytes). */
 int gq_profile_end(double *total_ms, int64_t *launches, char *name, int32_t name_len);
+/* As gq_profile_end, and *first_kernel_ms receives the summed time of each launch's first traversal
+ * kernel (the packet kernel, or the lane kernel of rays and lane-per-query silhouettes): the
+ * dominant kernel, timed on its own stream.  No counterpart in the reference (its demos time whole
+ * dr.eval calls, demo/closest_point_demo_3d.py:103-107). */
+int gq_profile_end_split(double *total_ms, double *first_kernel_ms, int64_t *launches, char *name,
+                         int32_t name_len);
 
 #ifdef __cplusplus
 }

@@ -77,3 +77,24 @@ def test_batch_rejects_bad_jobs(gpu):
         Batch([(shape, "cp", x)], method="plain")
     with pytest.raises(ValueError):
         Batch([{"shape": shape, "kind": "cp", "x": x, "traversal": 7}])
+
+
+def test_batch_input_order_equals_sorted(gpu):
+    """order=1 (the caller's order, no Morton sort) changes the wavefront grouping only: every
+    record equals the sorted batch's bit for bit (ties break to the lowest primitive slot)."""
+    from gquery_amd.batch import Batch
+    rng = np.random.default_rng(9)
+    jobs = []
+    for k, (name, kind, n) in enumerate([("ico4", "cp", 20000), ("star4096", "cp", 30000), ("oico4", "sil", 12000),
+                                          ("star4096", "sil", 9000)]):
+        V, F, shape = scene(name)
+        x = torch.from_numpy(synth.uniform_points(n, V.shape[1], seed=500 + k)).cuda()
+        jobs.append({"shape": shape, "kind": kind, "x": x})
+    sorted_ = Batch(jobs).run()
+    unsorted = Batch([dict(j, order=1) for j in jobs]).run()
+    for j, a, b in zip(jobs, sorted_, unsorted):
+        A, B = _np(a), _np(b)
+        for f in A:
+            assert np.array_equal(A[f].view(np.uint8), B[f].view(np.uint8)), (j["kind"], f)
+    with pytest.raises(ValueError):
+        Batch([dict(jobs[0], order=2)])
