@@ -108,6 +108,7 @@ SIGNATURES = {
     "gq_structure_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "gq_structure_geometry": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.c_void_p, _i64p]),
     "gq_profile_begin": (C.c_int, []),
+    "gq_closest_point_bvh_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_profile_end": (C.c_int, [C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
     "gq_profile_end_split": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), _i64p, C.c_char_p, C.c_int32]),
 }

@@ -1904,11 +1904,16 @@ bool sort_job(int kind, int64_t n) { return n >= kSortMin && kind != kIntersect;
 bool sort_job(const EngineJob &jb) { return !jb.input_order && sort_job(jb.kind, jb.q.n); }
 
 // Fills the plan for `jobs` with every device pointer relative to `base` (null when only the
-// sizes are wanted).
-hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &plan) {
-    plan = Plan{};
+// sizes are wanted).  tiles_implicit: the launch reads no tile table (a single job), so only the
+// tile counts are kept.  A plan object reused for the same shape of job list is refilled without
+// host allocation (its vectors keep their capacity).
+hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &plan, bool tiles_implicit = false) {
     const size_t nj = jobs.size();
-    plan.P.resize(nj);
+    plan.P.assign(nj, EngineParams{});
+    plan.tiles.clear();
+    plan.table_bytes = plan.ws_bytes = 0;
+    size_t n_groups = 0;
+    size_t n_tiles_all = 0;
     // groups
     for (size_t j = 0; j < nj; ++j) {
         const EngineJob &jb = jobs[j];
@@ -1917,41 +1922,54 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
                               (jb.traversal == 2 || (jb.traversal == 0 && w.dim == 2)) && w.bin_depth + 2 <= kRayStackMax;
         const bool lane = jb.kind == kIntersect || sil_lane;
         int g = -1;
-        for (size_t k = 0; k < plan.groups.size(); ++k) {
+        for (size_t k = 0; k < n_groups; ++k) {
             const Group &G = plan.groups[k];
             if (G.kind == jb.kind && G.dim == w.dim && G.quantized == jb.quantized && G.lane == lane) g = (int)k;
         }
         if (g < 0) {
-            Group G;
+            if (n_groups == plan.groups.size()) plan.groups.emplace_back();
+            Group &G = plan.groups[n_groups];
+            G.jobs.clear();
+            G.tiles_off = G.n_tiles = G.dc_off = G.def_off = G.ovf_off = 0;
+            G.defer = false;
+            G.max_stack = G.front = G.front_small = 0;
             G.kind = jb.kind;
             G.dim = w.dim;
             G.quantized = jb.quantized;
             G.lane = lane;
-            plan.groups.push_back(G);
-            g = (int)plan.groups.size() - 1;
+            g = (int)n_groups++;
         }
         plan.groups[g].jobs.push_back((int)j);
     }
+    plan.groups.resize(n_groups);
     // tile tables: sort tiles per dimension, then traversal tiles per group
+    auto add_tiles = [&](size_t j, int ts) {
+        const int64_t n = jobs[j].q.n;
+        if (tiles_implicit) {
+            n_tiles_all += (size_t)((n + ts - 1) / ts);
+            return;
+        }
+        for (int64_t t = 0; t < n; t += ts) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
+        n_tiles_all = plan.tiles.size();
+    };
     for (int di = 0; di < 2; ++di) {
-        plan.sort_tiles_off[di] = plan.tiles.size();
+        plan.sort_tiles_off[di] = n_tiles_all;
         for (size_t j = 0; j < nj; ++j) {
             if (jobs[j].w->dim != di + 2 || !sort_job(jobs[j])) continue;
-            for (int64_t t = 0; t < jobs[j].q.n; t += kBlock) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
+            add_tiles(j, kBlock);
         }
-        plan.n_sort_tiles[di] = plan.tiles.size() - plan.sort_tiles_off[di];
+        plan.n_sort_tiles[di] = n_tiles_all - plan.sort_tiles_off[di];
     }
     for (Group &G : plan.groups) {
         const int ts = G.lane ? kRayBlock : kPacketBlock;
-        G.tiles_off = plan.tiles.size();
-        for (int j : G.jobs)
-            for (int64_t t = 0; t < jobs[j].q.n; t += ts) plan.tiles.push_back(make_uint2((uint32_t)j, (uint32_t)t));
-        G.n_tiles = plan.tiles.size() - G.tiles_off;
+        G.tiles_off = n_tiles_all;
+        for (int j : G.jobs) add_tiles((size_t)j, ts);
+        G.n_tiles = n_tiles_all - G.tiles_off;
     }
     // workspace layout: [job table][tiles][per job scratch][per group deferred lists]
     size_t off = align256(nj * sizeof(EngineParams));
     const size_t tiles_at = off;
-    off = align256(off + plan.tiles.size() * sizeof(uint2));
+    off = align256(off + (tiles_implicit ? 0 : n_tiles_all) * sizeof(uint2));
     plan.table_bytes = off;
     auto at = [&](size_t bytes) {
         const size_t o = off;
@@ -2193,7 +2211,7 @@ hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, cons
     if (q.n <= 0) return hipSuccess;
     const std::vector<EngineJob> jobs{EngineJob{&w, kind, quantized, q, 0, w.traversal}};
     Plan plan;
-    hipError_t e = make_plan(jobs, nullptr, plan);
+    hipError_t e = make_plan(jobs, nullptr, plan, true);
     if (e != hipSuccess) return e;
     // A single call sorts with a device radix sort of the full Morton codes (fastest for one
     // large batch); batches use the counting sort, which records into graphs.
@@ -2211,7 +2229,7 @@ hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, cons
     void *buf = nullptr;
     if ((e = hipMallocAsync(&buf, plan.ws_bytes + extra, st)) != hipSuccess) return e;
     uint8_t *base = (uint8_t *)buf;
-    make_plan(jobs, base, plan);
+    make_plan(jobs, base, plan, true);
     const EngineParams &P = plan.P[0];
     hipLaunchKernelGGL(k_store_params, dim3(1), dim3(64), 0, st, P, (EngineParams *)base);
     const EngineParams *J = (const EngineParams *)base;
@@ -2234,6 +2252,53 @@ hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, cons
     }
     const hipError_t f = hipFreeAsync(buf, st);
     return e != hipSuccess ? e : f;
+}
+
+// ---- allocation-free single call ---------------------------------------------------------------
+// The scratch of gq_closest_point_bvh_noalloc: a device array of the code object, one copy per GPU,
+// held until the process ends.  Host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (the name itself is a host-side stand-in).  Every pass of the call
+// clears what it needs with a kernel (the counting sort's cell counters, the deferral counters), so
+// the array's contents from the previous call do not matter; calls that overlap in time on one GPU
+// share it and race.
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+// Address of `scratch` on the current device, looked up on the first call per device.
+static hipError_t scratch_on_device(uint8_t **out) {
+    static std::mutex mu;
+    static std::map<int, uint8_t *> ptrs;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = ptrs.find(dev);
+    if (it == ptrs.end()) {
+        void *p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+        it = ptrs.emplace(dev, (uint8_t *)p).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
+
+hipError_t launch_engine_noalloc(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q, hipStream_t st,
+                                 size_t *need) {
+    *need = 0;
+    if (q.n <= 0) return hipSuccess;
+    // host-side plan reused across calls on this thread: no host allocation once it has its size
+    thread_local std::vector<EngineJob> jobs(1);
+    thread_local Plan plan;
+    jobs[0] = EngineJob{&w, kind, quantized, q, 0, w.traversal};
+    hipError_t e = make_plan(jobs, nullptr, plan, true);
+    if (e != hipSuccess) return e;
+    *need = plan.ws_bytes;
+    if (plan.ws_bytes > kNoAllocScratchBytes) return hipErrorInvalidValue;
+    uint8_t *base = nullptr;
+    if ((e = scratch_on_device(&base)) != hipSuccess) return e;
+    make_plan(jobs, base, plan, true);
+    hipLaunchKernelGGL(k_store_params, dim3(1), dim3(64), 0, st, plan.P[0], (EngineParams *)base);
+    // the counting sort of the batch path (no library temporary storage), then the traversal
+    return launch_plan(plan, base, true, st, nullptr);
 }
 
 hipError_t engine_batch_sizes(const std::vector<EngineJob> &jobs, size_t &ws_bytes, size_t &table_bytes) {

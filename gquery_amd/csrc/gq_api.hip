@@ -389,6 +389,30 @@ int gq_closest_point_bvh(gq_scene *scene, int method, const float *x, int64_t n,
     return run_query(scene, kClosestPoint, method, q, stream);
 }
 
+int gq_closest_point_bvh_noalloc(gq_scene *s, int method, const float *x, int64_t n, const gq_records *out,
+                                 void *stream) {
+    if (!out) return fail(GQ_ERR_INVALID, "null records");
+    if (!s) return fail(GQ_ERR_INVALID, "null scene");
+    if (n < 0) return fail(GQ_ERR_INVALID, "negative query count");
+    if (n == 0) return GQ_OK;
+    if (n > (int64_t)INT32_MAX) return fail(GQ_ERR_INVALID, "more than 2^31-1 queries in one call");
+    if (!x) return fail(GQ_ERR_INVALID, "null query pointer");
+    if (method != GQ_METHOD_ENGINE && method != GQ_METHOD_ENGINE_FP32)
+        return fail(GQ_ERR_UNSUPPORTED, "the allocation-free call runs the engine methods only");
+    if (!engine_supported(kClosestPoint, s->wide))
+        return fail(GQ_ERR_UNSUPPORTED, "tree deeper than the engine kernels' traversal stack");
+    QueryArgs q{x, nullptr, nullptr, __builtin_inff(), n, *out};
+    size_t need = 0;
+    const hipError_t e = launch_engine_noalloc(kClosestPoint, s->wide, quantized_layout(method, kClosestPoint, s->wide), q,
+                                               (hipStream_t)stream, &need);
+    if (need > kNoAllocScratchBytes)
+        return fail(GQ_ERR_INVALID, "gq_closest_point_bvh_noalloc: " + std::to_string(n) + " queries need " +
+                                        std::to_string(need) + " bytes of scratch, more than the fixed " +
+                                        std::to_string(kNoAllocScratchBytes) + " (split the batch)");
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return GQ_OK;
+}
+
 int gq_intersect_bvh(gq_scene *scene, int method, const float *x, const float *v,
                      const float *r_max, float r_max_scalar, int64_t n, const gq_records *out,
                      void *stream) {

@@ -194,6 +194,12 @@ struct EngineJob {
 
 // One batch on `st`: scratch comes from the stream-ordered allocator and is released after use.
 hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q, hipStream_t st);
+// As launch_engine without allocating: scratch comes from a fixed device array of
+// kNoAllocScratchBytes (one per GPU).  *need receives the bytes the call needs; more than the array
+// holds gives hipErrorInvalidValue and launches nothing.
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
+hipError_t launch_engine_noalloc(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q, hipStream_t st,
+                                 size_t *need);
 // Many batches (any scenes, kinds and sizes on one device) in one enqueue: sort passes over all of
 // them, then one traversal launch per (kind, dimension, node format, traversal) group.  The job
 // table and tile lists are staged in `stage` (pinned host memory, `table_bytes` long, left

@@ -147,6 +147,16 @@ int gq_scene_info(const gq_scene *scene, int64_t info[8]);
 int gq_closest_point_bvh(gq_scene *scene, int method, const float *x, int64_t n,
                          const gq_records *out, void *stream);
 
+/* No counterpart in the reference: gq_closest_point_bvh (engine methods) without any device
+ * allocation, for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its scratch (sort keys, permutation, deferred-query lists) is a
+ * fixed 48 MiB device array held by each GPU the library runs on until the process ends: enough for
+ * 1,048,576 closest-point queries; larger calls fail with GQ_ERR_INVALID and launch nothing.  Calls
+ * that overlap in time on one GPU share that array and race: order them on one stream.  Results
+ * equal gq_closest_point_bvh's bit for bit. */
+int gq_closest_point_bvh_noalloc(gq_scene *scene, int method, const float *x, int64_t n,
+                                 const gq_records *out, void *stream);
+
 /* Polyline/Mesh.intersect_bvh (polyline.py:132-135, mesh.py:86-89) -> BVH.intersect
  * (bvh.py:200-276) / BVH3D.intersect (bvh3d.py:176-239).  r_max: per-query device array, or NULL
  * to use r_max_scalar for every ray.  Fills valid, p, n, uv/t, d, prim_id, on_boundary. */
