@@ -631,7 +631,7 @@ void vertex_faces(const int32_t *F, int64_t nf, int64_t nv, std::vector<int32_t>
 
 namespace {
 struct Fnv {
-    uint64_t h = 1469598103934665603ull;
+    uint64_t h = 14695981039346656037ull;   // FNV-1a 64-bit offset basis
     void add(const void *p, size_t n) {
         const uint8_t *b = (const uint8_t *)p;
         for (size_t k = 0; k < n; ++k) { h ^= b[k]; h *= 1099511628211ull; }
@@ -671,6 +671,88 @@ bool save_structure(const HostStructure &h, const char *path, std::string &err) 
     if (std::fclose(f) != 0) ok = false;
     if (!ok) err = std::string("write failed: ") + path;
     return ok;
+}
+
+// A float field that holds a non-negative integer (the wire format stores offsets and counts as
+// float32): exact, finite, below 2^24.
+static bool int_field(float v, int64_t &out) {
+    if (!(v >= 0.0f) || !(v < 16777216.0f) || v != std::floor(v)) return false;
+    out = (int64_t)v;
+    return true;
+}
+
+// The checks a loaded structure needs before any kernel indexes with it: every node of a tree is
+// reached exactly once from the root, child offsets and leaf ranges (primitives; silhouettes for
+// an SNCH) lie inside their arrays, integer fields are exact, and the deepest node matches the
+// stored max_depth (the plain kernels' stack bound and the engine's depth checks read it).
+static bool check_tree(const std::vector<float> &nd, int stride, int64_t n_nodes, int off_f, int second_f,
+                       int count_f, int64_t n_items, int sil_off_f, int sil_cnt_f, int64_t n_sils, int32_t &depth,
+                       std::string &err) {
+    std::vector<uint8_t> seen((size_t)n_nodes, 0);
+    std::vector<std::pair<int64_t, int32_t>> stack{{0, 0}};
+    int64_t visited = 0;
+    depth = 0;
+    while (!stack.empty()) {
+        const auto [i, d] = stack.back();
+        stack.pop_back();
+        if (i < 0 || i >= n_nodes || seen[(size_t)i]) { err = "node reached twice or out of range"; return false; }
+        seen[(size_t)i] = 1;
+        ++visited;
+        depth = std::max(depth, d);
+        const float *n = nd.data() + (size_t)i * stride;
+        int64_t off, second, count;
+        if (!int_field(n[off_f], off) || !int_field(n[second_f], second) || !int_field(n[count_f], count)) {
+            err = "non-integer offset or count in node " + std::to_string(i);
+            return false;
+        }
+        if (sil_off_f >= 0) {
+            int64_t so, sc;
+            if (!int_field(n[sil_off_f], so) || !int_field(n[sil_cnt_f], sc) || so + sc > n_sils) {
+                err = "silhouette range out of bounds in node " + std::to_string(i);
+                return false;
+            }
+        }
+        if (count > 0) {
+            if (off + count > n_items) { err = "leaf range out of bounds in node " + std::to_string(i); return false; }
+        } else {
+            if (second < 2 || i + second >= n_nodes) { err = "child offset out of bounds in node " + std::to_string(i); return false; }
+            stack.push_back({i + second, d + 1});
+            stack.push_back({i + 1, d + 1});
+        }
+    }
+    if (visited != n_nodes) { err = "nodes unreachable from the root"; return false; }
+    return true;
+}
+
+static bool check_loaded(const HostStructure &s, std::string &err) {
+    const int D = s.dim;
+    int32_t depth = 0;
+    if (!check_tree(s.bvh_node_data, kBvhNodeStride[D], s.n_nodes, 2 * D, 2 * D + 1, 2 * D + 2, s.n_prims, -1, -1, 0,
+                    depth, err))
+        return false;
+    if (depth != s.max_depth) { err = "stored max_depth does not match the tree"; return false; }
+    if (s.is_snch) {
+        // SNCH node: box, cone axis, half-angle, radius, then prim_off, second_child, n_prims,
+        // sil_off, n_sils (snch.h:44-50)
+        const int b = 2 * D + D + 2;
+        int32_t sd = 0;
+        if (!check_tree(s.snch_node_data, kSnchNodeStride[D], s.n_nodes, b, b + 1, b + 2, s.n_prims, b + 3, b + 4,
+                        s.n_silhouettes, sd, err))
+            return false;
+    }
+    const int64_t nv = (int64_t)(s.vertices.size() / (size_t)D);
+    for (int32_t v : s.indices)
+        if (v < 0 || v >= nv) { err = "vertex index out of range"; return false; }
+    for (int32_t v : s.ordered_prim_index)
+        if (v < 0 || v >= s.n_prims) { err = "primitive index out of range"; return false; }
+    for (int64_t k = 0; k < s.n_prims; ++k) {
+        int64_t id;
+        if (!int_field(s.primitive_data[(size_t)(k * kPrimStride[D] + kPrimStride[D] - 1)], id) || id >= s.n_prims) {
+            err = "primitive record id out of range";
+            return false;
+        }
+    }
+    return true;
 }
 
 bool load_structure(const char *path, HostStructure &h, std::string &err) {
@@ -723,6 +805,8 @@ bool load_structure(const char *path, HostStructure &h, std::string &err) {
     if (ok && c != want) { err = std::string("checksum mismatch: ") + path; return false; }
     if (ok && s.vertices.size() % (size_t)D != 0) ok = false;
     if (!ok) { err = std::string("not a valid scene file: ") + path; return false; }
+    std::string why;
+    if (!check_loaded(s, why)) { err = std::string("inconsistent scene file (") + why + "): " + path; return false; }
     h = std::move(s);
     return true;
 }

@@ -10,10 +10,17 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
+from .fwd import SoA
 
 
 def _as_rows(a, dim: int, dtype) -> np.ndarray:
-    """(N, dim) C-contiguous array from numpy/torch/sequence input; (dim, N) is transposed."""
+    """(N, dim) C-contiguous array from fwd.Array2 / Array3 / Array2i / Array3i (components first,
+    as the reference's DrJit arrays) or numpy / torch / sequence rows.  Untagged (dim, N) input is
+    transposed only where that is unambiguous (N != dim): a square array is read as rows."""
+    if isinstance(a, SoA):
+        if a.dim != dim:
+            raise ValueError(f"expected {dim}-component arrays, got {type(a).__name__}")
+        a = a.rows()
     try:
         import torch
         if isinstance(a, torch.Tensor):

@@ -12,6 +12,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .fwd import SoA
 from .shapes import primitive as rec
 
 INF = float("inf")
@@ -26,6 +27,14 @@ def _device(device=None) -> torch.device:
 
 
 def as_points(x, dim: int, device: torch.device) -> torch.Tensor:
+    """(N, dim) float32 points on `device`.  Input: fwd.Array2 / Array3 (components first, as the
+    reference's DrJit arrays), or numpy / torch / sequences in (N, dim) rows.  Untagged input in
+    (dim, N) layout is accepted only where that is unambiguous (N != dim); a square (dim, dim)
+    array is read as rows -- pass components-first data of that shape as Array2 / Array3."""
+    if isinstance(x, SoA):
+        if x.dim != dim:
+            raise ValueError(f"expected {dim}-component points, got {type(x).__name__}")
+        x = x.rows()
     if not isinstance(x, torch.Tensor):
         x = torch.as_tensor(np.asarray(x, dtype=np.float32))
     if x.ndim == 1 and x.shape[0] == dim:
