@@ -43,7 +43,10 @@ class Batch:
     packet_budget / traversal override the scene's settings for that job (0 = scene's); order=1
     traverses the job in the caller's order (no Morton sort)."""
 
-    def __init__(self, jobs, method: str = "engine", device=None):
+    def __init__(self, jobs, method: str = "engine", device=None, workspace=None, staging=None):
+        """workspace / staging: optional reusable buffers (a uint8 device tensor and a pinned uint8
+        host tensor) used instead of fresh ones when they are large enough; the caller must not
+        hand them to another batch until this one's enqueue has completed on its stream."""
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self._keep = []
         self.results = []
@@ -82,8 +85,14 @@ class Batch:
         dev, host = C.c_int64(), C.c_int64()
         _lib.check(_lib.lib().gq_batch_sizes(self._n, arr, C.byref(dev), C.byref(host)), "gq_batch_sizes")
         self.device_bytes, self.host_bytes = dev.value, host.value
-        self._ws = torch.empty(max(self.device_bytes, 1), dtype=torch.uint8, device=self.device)
-        self._stage = torch.empty(max(self.host_bytes, 1), dtype=torch.uint8, pin_memory=True)
+        if workspace is not None and workspace.numel() >= self.device_bytes and workspace.device == self.device:
+            self._ws = workspace
+        else:
+            self._ws = torch.empty(max(self.device_bytes, 1), dtype=torch.uint8, device=self.device)
+        if staging is not None and staging.numel() >= self.host_bytes and staging.is_pinned():
+            self._stage = staging
+        else:
+            self._stage = torch.empty(max(self.host_bytes, 1), dtype=torch.uint8, pin_memory=True)
         self._graph = None
 
     def run(self):
@@ -114,6 +123,27 @@ class Batch:
             raise RuntimeError("capture() first")
         self._graph.replay()
         return self.results
+
+    @staticmethod
+    def sizes(jobs, method: str = "engine"):
+        """(device workspace bytes, pinned staging bytes) gq_batch_sizes reports for `jobs`, without
+        allocating records or buffers.  jobs: (shape or Scene, kind, x[, v]) with x, v on the device."""
+        arr = (_lib.Job * max(len(jobs), 1))()
+        for i, j in enumerate(jobs):
+            shape, kind, x = j[0], j[1], j[2]
+            v = j[3] if len(j) > 3 else None
+            scene = getattr(shape, "scene", shape)
+            jb = arr[i]
+            jb.scene = scene._h
+            jb.kind = _lib.QUERIES[kind]
+            jb.method = _lib.METHODS[method]
+            jb.x = x.data_ptr()
+            jb.v = v.data_ptr() if v is not None else None
+            jb.r_max_scalar = INF
+            jb.n = int(x.shape[0])
+        dev, host = C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().gq_batch_sizes(len(jobs), arr, C.byref(dev), C.byref(host)), "gq_batch_sizes")
+        return dev.value, host.value
 
     def __len__(self):
         return self._n

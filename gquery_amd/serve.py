@@ -33,6 +33,7 @@ class Request:
     admitted: Optional[float] = None
     done: Optional[float] = None
     result: Any = None
+    footprint: Optional[int] = None   # device bytes charged while in flight (set on submission)
 
     @property
     def n(self) -> int:
@@ -51,8 +52,9 @@ def record_bytes(kind: str, dim: int) -> int:
 
 
 def request_bytes(req: Request, dim: int) -> int:
-    """Device footprint of a request while in flight: inputs, records and scratch (sort arrays,
-    deferred list and tile tables: at most 48 bytes per query)."""
+    """A size-only estimate of a request's device footprint while in flight (inputs, records and
+    48 bytes of scratch per query), for executors that cannot ask the library; GpuExecutor charges
+    the records plus the workspace gq_batch_sizes reports instead (GpuExecutor.footprint)."""
     inputs = 4 * dim * req.n * (2 if req.kind == "ray" else 1)
     return inputs + record_bytes(req.kind, dim) * req.n + 48 * req.n + 4096
 
@@ -86,8 +88,16 @@ class Scheduler:
         self.in_use = 0
         self.completed: List[Request] = []
 
+    def footprint(self, req: Request) -> int:
+        """Device bytes charged for a request: the executor's figure where it has one (records plus
+        the workspace the library reports), else the size-only estimate.  Computed once per request."""
+        if req.footprint is None:
+            f = getattr(self.ex, "footprint", None)
+            req.footprint = int(f(req)) if f is not None else request_bytes(req, self.ex.dims[req.scene])
+        return req.footprint
+
     def submit(self, req: Request):
-        fp = request_bytes(req, self.ex.dims[req.scene])
+        fp = self.footprint(req)
         if fp > self.budget:
             raise MemoryError(f"request {req.rid} needs {fp} bytes, more than the whole budget {self.budget}")
         if req.kind not in self.classes:
@@ -117,7 +127,7 @@ class Scheduler:
                 batch, used, nq = [], 0, 0
                 while q:
                     r = q[0]
-                    fp = request_bytes(r, self.ex.dims[r.scene])
+                    fp = self.footprint(r)
                     if self.in_use + used + fp > self.budget or (batch and nq + r.n > self.max_batch_queries):
                         break
                     batch.append(q.popleft())
@@ -138,7 +148,10 @@ class Scheduler:
 
 
 class GpuExecutor:
-    """Launches a list of requests as one gquery_amd.batch.Batch on the next stream slot."""
+    """Launches a list of requests as one gquery_amd.batch.Batch on a free stream slot of its class.
+    A request is charged its records plus the batch workspace gq_batch_sizes reports for it alone
+    (the coalesced batch needs at most the sum: it shares the job and tile tables).  Workspaces and
+    pinned staging buffers are pooled and reused once the launch that last used them has completed."""
 
     def __init__(self, shapes, slots=(2, 1), method: str = "engine", device=None):
         import torch
@@ -150,24 +163,59 @@ class GpuExecutor:
         slots = (slots,) if isinstance(slots, int) else tuple(slots)
         # one stream per launch slot of each class, default priority (no priority argument)
         self.streams = [[torch.cuda.Stream(self.device) for _ in range(n)] for n in slots]
+        self._stream_ev = [[None] * n for n in slots]   # last launch's completion event per stream
         self._turn = [0] * len(slots)
+        self._buffers = []   # [workspace, staging, completion event of the last launch using them]
         self.launches = 0
+        self.buffer_allocations = 0
+
+    def footprint(self, req) -> int:
+        from .batch import Batch
+        dev, _ = Batch.sizes([(self.shapes[req.scene], req.kind, req.x, req.v)], method=self.method)
+        return record_bytes(req.kind, self.dims[req.scene]) * req.n + dev
+
+    def _take_buffers(self, dev_bytes: int, host_bytes: int):
+        """A pooled (workspace, staging) pair no launch is using, grown if too small."""
+        torch = self.torch
+        free = [b for b in self._buffers if b[2] is None or b[2].query()]
+        for b in free:
+            if b[0].numel() >= dev_bytes and b[1].numel() >= host_bytes:
+                return b
+        if free:
+            b = free[0]
+        else:
+            b = [None, None, None]
+            self._buffers.append(b)
+        if b[0] is None or b[0].numel() < dev_bytes:
+            b[0] = torch.empty(max(dev_bytes, 1), dtype=torch.uint8, device=self.device)
+            self.buffer_allocations += 1
+        if b[1] is None or b[1].numel() < host_bytes:
+            b[1] = torch.empty(max(host_bytes, 1), dtype=torch.uint8, pin_memory=True)
+            self.buffer_allocations += 1
+        return b
 
     def launch(self, reqs, cls: int = 0):
         from .batch import Batch
         torch = self.torch
         pool = self.streams[cls]
-        st = pool[self._turn[cls] % len(pool)]
+        # a stream of the class whose previous launch has finished, else the next in turn
+        k = next((i for i, e in enumerate(self._stream_ev[cls]) if e is None or e.query()),
+                 self._turn[cls] % len(pool))
         self._turn[cls] += 1
+        st = pool[k]
         self.launches += 1
+        jobs = [(self.shapes[r.scene], r.kind, r.x, r.v) for r in reqs]
+        dev, host = Batch.sizes(jobs, method=self.method)
+        buf = self._take_buffers(dev, host)
         # the request inputs were written on the default stream: order this launch after them
         st.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(st):
-            b = Batch([(self.shapes[r.scene], r.kind, r.x, r.v) for r in reqs], method=self.method,
-                      device=self.device)
+            b = Batch(jobs, method=self.method, device=self.device, workspace=buf[0], staging=buf[1])
             b.run()
             ev = torch.cuda.Event()
             ev.record(st)
+        buf[2] = ev
+        self._stream_ev[cls][k] = ev
         return (b, ev, reqs)
 
     def done(self, h) -> bool:

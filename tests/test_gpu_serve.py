@@ -42,3 +42,12 @@ def test_results_alone_equal_results_in_the_mix(gpu):
     lat = [r.latency for r in reqs]
     assert all(l > 0 for l in lat) and percentile(lat, 50) <= percentile(lat, 99)
     assert ex.launches < len(reqs)   # requests were coalesced into fewer launches
+    # workspaces and staging buffers were reused across launches, not allocated per launch
+    assert ex.buffer_allocations < 2 * ex.launches
+    # admission charged what the library reports: records + the request's own batch workspace
+    from gquery_amd.batch import Batch
+    from gquery_amd.serve import record_bytes
+    for r in reqs[:8]:
+        dev, _ = Batch.sizes([(shapes[r.scene], r.kind, r.x, r.v)])
+        dim = 2 if "star" in names[r.scene] else 3
+        assert r.footprint == record_bytes(r.kind, dim) * r.n + dev and dev > 0

@@ -154,3 +154,31 @@ def test_silhouettes_have_their_own_queue_and_slots():
     clk.t = 2.0
     s.poll()
     assert sorted(r.rid for r in s.completed) == [0, 1, 2, 3]
+
+
+class SizedExecutor(FakeExecutor):
+    """An executor that reports each request's device footprint (as GpuExecutor does from
+    gq_batch_sizes): 8 bytes per query of records plus a fixed 1 MB of workspace."""
+
+    def footprint(self, r):
+        return 8 * r.n + (1 << 20)
+
+
+def test_admission_charges_the_footprint_the_executor_reports():
+    clk = FakeClock()
+    ex = SizedExecutor(clk)
+    # room for exactly three reported footprints of 1000-query requests; the size-only estimate of
+    # the same requests is far smaller, so admitting by it would take all ten at once
+    fp = 8 * 1000 + (1 << 20)
+    assert request_bytes(req(0, 1000), 2) * 10 < 3 * fp
+    s = Scheduler(ex, memory_budget=3 * fp + 100, slots=4, clock=clk)
+    for i in range(10):
+        s.submit(req(i, 1000))
+    assert all(r.footprint == fp for q in s.pending for r in q)
+    s.poll()
+    assert [len(h["reqs"]) for h in ex.launches] == [3] and s.in_use == 3 * fp
+    clk.t = 1.0
+    s.poll()
+    assert s.in_use == 3 * fp and len(s.completed) == 3   # the first three freed, the next three in
+    with pytest.raises(MemoryError):
+        s.submit(req(99, 10 ** 6))                          # reported footprint above the whole budget
