@@ -32,9 +32,12 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # VALU issue peak of MI355X in wave64 instructions per second.  The published FP32 vector peak,
-# 157.3 TFLOP/s, counts packed FMAs: 256 CUs x 4 SIMDs x 2.4 GHz x 16 lanes x 2 (FMA) x 2 (packed)
-# per clock.  A wave64 VALU instruction therefore takes a SIMD 4 clocks: 1,024 SIMDs x 2.4e9 / 4 =
-# 614.4e9 instructions per second, the bound of kernels that are limited by VALU issue.
+# 157.3 TFLOP/s, is packed FMA (v_pk_fma_f32: 2 FMAs per lane) at one wave64 instruction per SIMD
+# every 4 clocks: 1,024 SIMDs x 2.4e9 / 4 = 614.4e9 instructions per second.  The probe built with
+# the library (gquery_amd/valu_probe, source csrc/valu_probe.hip; profiles/valu_issue_rate.md)
+# measures the same 4-clock cadence for v_fma_f32 and about 2 clocks for v_sub_f32 / v_max_f32, so
+# instructions x 4 clocks over the SIMDs' clocks is an upper bound on the VALU busy fraction (exact
+# for an FMA-only stream, twice too high for an add/max-only one).
 VALU_PEAK_GIPS = 157.3e12 / (64 * 2 * 2) / 1e9
 PATH_STEPS = 10
 PLAIN_STEPS = 3
@@ -173,6 +176,9 @@ def valu_roofline(method, packet_ms, traversal_ms, kernel_name):
             out["valu_instr_per_launch"] = valu
             out["achieved"] = valu / (packet_ms * 1e-3) / 1e9
             out["frac"] = out["achieved"] / VALU_PEAK_GIPS
+            out["issue_model"] = ("4 clocks per wave64 VALU instruction (the FMA rate); add/max/compare "
+                                  "issue in about 2, so frac bounds the VALU busy fraction from above and "
+                                  "frac/2 from below")
             out["counters"] = f"profiles/{cf.name}"
             out["counters_match_build"] = c.get("engine_src_sha") == _engine_sha()
     tf = ROOT / "profiles" / f"traffic_cp3d_s7_1M_{method}.json"

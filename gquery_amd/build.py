@@ -68,6 +68,10 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp)] + objs)
     os.replace(tmp, LIB)
+    # the VALU issue-rate probe behind bench.py's roofline peak (a separate program)
+    probe = CSRC / "valu_probe.hip"
+    if probe.exists():
+        _run([HIPCC, str(probe), "-o", str(PKG / "valu_probe")] + [f for f in HIP_FLAGS if f != "-fPIC"])
     return LIB
 
 
