@@ -113,6 +113,8 @@ struct EngineParams {
     uint32_t *defer_count;    // [0] deferred queries, [1] work counter of the wave-per-query pass
     uint4 *deferred;          // (query index, best distance bits, best slot, job)
     int sil_lane;             // silhouettes: lane-per-query kernel instead of packet + wave pass
+    const Seed *seed;         // seed cut the first wave-per-query pass starts from (n_seed = 0: the root)
+    int n_seed;
     gq_records out;
 };
 
@@ -143,6 +145,8 @@ struct GP {
     int budget;
     GQ_G uint32_t *defer_count;
     GQ_G uint4 *deferred;
+    const GQ_G Seed *seed;
+    int n_seed;
     GRecords out;
 };
 template <typename T> __device__ __forceinline__ GQ_G T *gcast(T *p) { return (GQ_G T *)p; }
@@ -191,6 +195,8 @@ __device__ __forceinline__ GP gview(const EngineParams &in) {
     P.budget = in.budget;
     P.defer_count = gcast(in.defer_count);
     P.deferred = gcast(in.deferred);
+    P.seed = gcast(in.seed);
+    P.n_seed = in.n_seed;
     P.out.valid = gcast(in.out.valid);
     P.out.p = gcast(in.out.p);
     P.out.n = gcast(in.out.n);
@@ -740,6 +746,37 @@ __device__ __forceinline__ void wave_handoff(uint4 *list, uint32_t *count, uint4
     stg(gcast(list) + o, entry);
 }
 
+// The first wave-per-query pass starts from the scene's seed cut instead of the root: lane k puts
+// seed k on the frontier, keyed as an opened child would be (closest points: squared box distance;
+// silhouettes: box distance).  Skipping the tests of the seeds' ancestors is exact, as the cut
+// never passes below a node whose reference cone test can cull and the box tests and exact culls
+// it skips are conservative.  The full-frontier pass keeps the root, where its frontier bound holds.
+template <int D, bool Squared>
+__device__ __forceinline__ int seed_front(const GP &P, const float *xc, uint2 *front) {
+    const int lane = threadIdx.x & 63;
+    const int n = P.n_seed;
+    if (lane < n) {
+        const GQ_G float4 *sp = (const GQ_G float4 *)(P.seed + lane);
+        const float4 a = ldg(sp), b = ldg(sp + 1);
+        float key;
+        if constexpr (Squared) {
+            const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+            key = 0.0f;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const float o = fmaxf(fmaxf(lo[c] - xc[c], xc[c] - hi[c]), 0.0f);
+                key = fmaf(o, o, key);
+            }
+        } else {
+            float dx;
+            if constexpr (D == 3) box_distance(v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(xc[0], xc[1], xc[2]), key, dx);
+            else box_distance(v2(a.x, a.y), v2(b.x, b.y), v2(xc[0], xc[1]), key, dx);
+        }
+        front[lane] = make_uint2(__float_as_uint(a.w), __float_as_uint(key));
+    }
+    return n;
+}
+
 template <int D, bool Q>
 __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uint2 *front, int cap, uint4 *ovf,
                                              uint32_t *ovf_n) {
@@ -757,7 +794,8 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
         float bound2 = best < kInf ? cp_bound2(best, slack) : kInf;
         int size = 1;
         bool overflow = false;
-        front[0] = make_uint2(P.root, 0u);
+        if (ovf != nullptr && P.n_seed > 0 && P.n_seed <= cap) size = seed_front<D, true>(P, pc, front);
+        else front[0] = make_uint2(P.root, 0u);
         while (size > 0) {
             const int take = size < 64 ? size : 64;
             uint2 e = make_uint2(0u, __float_as_uint(kInf));
@@ -1461,8 +1499,12 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
         int size = 0;
         bool overflow = false;
         if (r >= 0.0f) {
-            front[0] = make_uint2(P.root, 0u);
-            size = 1;
+            if (ovf != nullptr && P.n_seed > 0 && P.n_seed <= cap) {
+                size = seed_front<D, false>(P, xc, front);
+            } else {
+                front[0] = make_uint2(P.root, 0u);
+                size = 1;
+            }
         }
         while (size > 0) {
             const int take = size < 64 ? size : 64;
@@ -1997,11 +2039,15 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             P.ref_cones = w.ref_cones;
             P.root = w.sroot_ref;
             for (int k = 0; k < 3; ++k) { P.root_lo[k] = w.sroot_lo[k]; P.root_hi[k] = w.sroot_hi[k]; }
+            P.seed = (const Seed *)w.sil_seed;
+            P.n_seed = w.n_sil_seed;
         } else {
             P.nodes = jb.quantized ? w.qnodes : w.nodes;
             P.prims = w.prims;
             P.ids = w.prim_ids;
             P.root = w.root_ref;
+            P.seed = (const Seed *)w.bvh_seed;
+            P.n_seed = w.n_bvh_seed;
         }
         if (sort_job(jb)) {
             P.cell_bits = cell_bits_for(jb.q.n, w.dim);

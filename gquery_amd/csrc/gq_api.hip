@@ -323,6 +323,10 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     if ((rc = upload(s.get(), wh.prims.data(), wh.prims.size(), &wv.prims))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.prim_ids.data(), wh.prim_ids.size(), &wv.prim_ids))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.big_leaves.data(), wh.big_leaves.size(), &wv.big_leaves))) return cleanup(rc);
+    wv.n_bvh_seed = (int)wh.bvh_seed.size();
+    wv.n_sil_seed = (int)wh.sil_seed.size();
+    if ((rc = upload(s.get(), wh.bvh_seed.data(), wh.bvh_seed.size(), (const Seed **)&wv.bvh_seed))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.sil_seed.data(), wh.sil_seed.size(), (const Seed **)&wv.sil_seed))) return cleanup(rc);
     if (h.is_snch) {
         if ((rc = upload(s.get(), wh.snodes.data(), wh.snodes.size(), (const uint8_t **)&wv.snodes))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.qsnodes.data(), wh.qsnodes.size(), (const uint8_t **)&wv.qsnodes))) return cleanup(rc);

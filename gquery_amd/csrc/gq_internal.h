@@ -42,6 +42,8 @@ struct WideView {
     const int32_t *sil_ids;
     const RefCone *ref_cones;
     const uint32_t *big_leaves;
+    const void *bvh_seed, *sil_seed;   // Seed[n_bvh_seed], Seed[n_sil_seed]
+    int n_bvh_seed, n_sil_seed;
     uint32_t root_ref, sroot_ref;
     float sroot_lo[3], sroot_hi[3];
     float scene_lo[3], scene_hi[3];

@@ -503,6 +503,66 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
     for (int k = 0; k < D; ++k) { out.sroot_lo[k] = b[0].lo[k]; out.sroot_hi[k] = b[0].hi[k]; }
 }
 
+// Seed cut of a tree of wide records: breadth first from the root's children, each round opens
+// every interior entry (in list order) while the cut stays within kSeedMax entries.  `children`
+// gives a record's (ref, box) children and whether the cut may pass below that record.
+template <int D, typename Children>
+std::vector<Seed> seed_cut(uint32_t root, Children children) {
+    std::vector<Seed> cut, next;
+    if (ref_is_leaf(root)) return cut;
+    std::vector<Seed> kids;
+    if (!children(root, kids)) return cut;
+    cut = kids;
+    for (bool opened = true; opened;) {
+        opened = false;
+        next.clear();
+        for (size_t k = 0; k < cut.size(); ++k) {
+            const Seed &e = cut[k];
+            kids.clear();
+            if (!ref_is_leaf(e.ref) && children(e.ref, kids) &&
+                next.size() + kids.size() + (cut.size() - k - 1) <= (size_t)kSeedMax) {
+                next.insert(next.end(), kids.begin(), kids.end());
+                opened = true;
+            } else {
+                next.push_back(e);
+            }
+        }
+        cut.swap(next);
+    }
+    return cut;
+}
+
+inline Seed make_seed(uint32_t ref, const float *lo, const float *hi, int d) {
+    Seed e;
+    std::memset(&e, 0, sizeof e);
+    e.ref = ref;
+    for (int k = 0; k < d; ++k) { e.lo[k] = lo[k]; e.hi[k] = hi[k]; }
+    return e;
+}
+
+template <int D> void build_seeds(WideHost &out) {
+    out.bvh_seed = seed_cut<D>(out.root_ref, [&](uint32_t ref, std::vector<Seed> &kids) {
+        FNode<D> nd;
+        std::memcpy(&nd, out.nodes.data() + (size_t)ref * sizeof nd, sizeof nd);
+        for (uint32_t j = 0; j < nd.nchild; ++j) {
+            float lo[3], hi[3];
+            for (int k = 0; k < D; ++k) { lo[k] = nd.lo[k][j]; hi[k] = nd.hi[k][j]; }
+            kids.push_back(make_seed(nd.child[j], lo, hi, D));
+        }
+        return true;
+    });
+    if (out.n_snch_nodes == 0) return;
+    // below a child whose reference cone test can cull, every query must still take that test,
+    // so the cut stops at such a pair (the box tests and exact culls it skips are conservative)
+    out.sil_seed = seed_cut<D>(out.sroot_ref, [&](uint32_t ref, std::vector<Seed> &kids) {
+        SPairF<D> pf;
+        std::memcpy(&pf, out.snodes.data() + (size_t)ref * sizeof pf, sizeof pf);
+        if (pf.ref_cone[0] >= 0 || pf.ref_cone[1] >= 0) return false;
+        for (int j = 0; j < 2; ++j) kids.push_back(make_seed(pf.child[j], pf.lo[j], pf.hi[j], D));
+        return true;
+    });
+}
+
 template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_leaf) {
     std::vector<BinNode> b = read_bvh<D>(h);
     LeafTable lt;
@@ -538,6 +598,7 @@ template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_
     }
     if (h.is_snch) build_snch_pairs<D>(h, b, out, lt);
     out.big_leaves = lt.big;
+    build_seeds<D>(out);
 }
 
 } // namespace

@@ -161,6 +161,18 @@ struct RefCone {
     float lo[3], hi[3];   // the node's box (the reference test uses its centroid and p_max)
 };
 
+// One entry of a seed cut: a node (child ref) below the root and its float32 box.  The
+// wave-per-query passes start from the cut (the root's descendants at a fixed depth, as far as the
+// reference cone tests allow) instead of from the root.
+constexpr int kSeedMax = 64;
+struct alignas(32) Seed {
+    float lo[3];
+    uint32_t ref;
+    float hi[3];
+    float pad;
+};
+static_assert(sizeof(Seed) == 32, "Seed is two float4");
+
 struct WideHost {
     int width = 4;
     int wide_depth = 0;
@@ -176,6 +188,7 @@ struct WideHost {
     std::vector<int32_t> sil_ids;
     std::vector<RefCone> ref_cones;
     std::vector<uint32_t> big_leaves;  // (first, count) pairs of leaves with more than 126 entries
+    std::vector<Seed> bvh_seed, sil_seed;   // seed cuts of the BVH4 and of the SNCH pairs
 };
 
 void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims);
