@@ -219,7 +219,9 @@ def query_paths(mesh, poly, x3, v3, x2, v2, log):
     from gquery_amd.batch import Batch
     from gquery_amd.shapes.mesh import Mesh
     nq = x3.shape[0]
+    from gquery_amd.shapes.polyline import Polyline
     mesh_q8 = Mesh(None, None, device="cuda:0", structure=mesh.snch, sil_pairs=1)
+    poly_q16 = Polyline(None, None, device="cuda:0", structure=poly.snch, sil_pairs=1)
     cases = {
         "cp3_engine_fp32": lambda: mesh.closest_point_bvh(x3, method="engine_fp32"),
         "ray3": lambda: mesh.intersect_bvh(x3, v3),
@@ -229,6 +231,7 @@ def query_paths(mesh, poly, x3, v3, x2, v2, log):
         "ray2": lambda: poly.intersect_bvh(x2, v2),
         "sil2": lambda: poly.closest_silhouette_snch(x2),
         "sil2_engine_fp32": lambda: poly.closest_silhouette_snch(x2, method="engine_fp32"),
+        "sil2_pairs_q16": lambda: poly_q16.closest_silhouette_snch(x2),
     }
     b_unsorted = Batch([{"shape": mesh, "kind": "cp", "x": x3, "order": 1}])
     cases["cp3_input_order"] = b_unsorted.run
@@ -261,6 +264,7 @@ def accuracy(args):
     for name, dim, V, F in scenes:
         shape = (Mesh if dim == 3 else Polyline)(V, F, device="cuda:0")
         s = shape.snch
+        shape_q = (Mesh if dim == 3 else Polyline)(None, None, device="cuda:0", structure=s, sil_pairs=1)
         x = synth.uniform_points(n, dim, seed=101)
         v = synth.uniform_directions(n, dim, seed=102)
         for kind in ("cp", "ray", "sil"):
@@ -281,11 +285,15 @@ def accuracy(args):
             with np.errstate(invalid="ignore"):
                 gap = bvalid & (~np.isfinite(b2) | (b2 - bd > 1e-5 * np.maximum(bd, 1e-30)))
             log(f"{name} {kind}: oracle + FP64 brute force {time.perf_counter() - t0:.1f}s")
-            for method in ("plain", "engine_fp32", "engine"):
+            methods = ("plain", "engine_fp32", "engine") + (("engine_quantized_pairs",) if kind == "sil" else ())
+            for method in methods:
                 if kind == "cp":
                     r = shape.closest_point_bvh(x, method=method)
                 elif kind == "ray":
                     r = shape.intersect_bvh(x, v, method=method)
+                elif method == "engine_quantized_pairs":
+                    # the quantized pair records (8-bit boxes in 3D, 16-bit in 2D, int16 cones)
+                    r = shape_q.closest_silhouette_snch(x, method="engine")
                 else:
                     r = shape.closest_silhouette_snch(x, method=method)
                 torch.cuda.synchronize()
@@ -310,6 +318,12 @@ def accuracy(args):
                        "p99_err_d_vs_fp64": float(np.percentile(rel64[same], 99)) if same.any() else 0.0,
                        "max_err_d_vs_fp32_restatement": float(rel32[rv].max()) if rv.any() else 0.0,
                        "mean_err_d_vs_fp32_restatement": float(rel32[rv].mean()) if rv.any() else 0.0}
+                if kind == "sil":
+                    # where the reference traversal itself (its cone test) misses the FP64 nearest
+                    # silhouette; the engine follows the reference there
+                    ok_ref = ref["valid"].astype(bool) & bvalid & gap
+                    out["reference_misses_fp64_nearest"] = float((ok_ref & (ref["prim_id"] != bf["prim_id"])).sum()
+                                                                 / max(ok_ref.sum(), 1))
                 if "p" in bf:
                     # point error: Euclidean distance to the FP64 point, over max(d, 1) like d
                     gp = r.p.cpu().numpy().astype(np.float64).reshape(n, dim)

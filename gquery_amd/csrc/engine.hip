@@ -1017,8 +1017,63 @@ struct PairChild {
 
 template <int D, bool Q>
 __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairChild (&c)[2]) {
-    if constexpr (D == 2) {
-        // SPairF<2> (192 B, both node formats): lo[2][2] @0, hi[2][2] @16, cone axis @32, half @48,
+    if constexpr (Q) {
+        // SPairQ<D> (64 B): origin @0, exp @4D, flags, child boxes (u8 in 3D, u16 in 2D), exact-cull
+        // cone axis (int16) @28, its sine (int16), child refs, ref_base, silhouette extents (slab
+        // middle int16, half-width u8, perpendicular u8)
+        constexpr int kFlags = D == 3 ? 15 : 10, kLo = D == 3 ? 16 : 12, kHi = D == 3 ? 22 : 20;
+        constexpr int kAx = 28, kSin = D == 3 ? 40 : 36, kChild = D == 3 ? 44 : 40, kRef = D == 3 ? 52 : 48;
+        constexpr int kExt = D == 3 ? 56 : 52;
+        Words<16> r;
+        r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64);
+        const uint32_t flags = r.u8(kFlags);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float org = r.f32(4 * k);
+            const float sc = ldexpf(1.0f, r.s8(4 * D + k));
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if constexpr (D == 3) {
+                    c[j].lo[k] = fmaf((float)r.u8(kLo + 3 * j + k), sc, org);
+                    c[j].hi[k] = fmaf((float)r.u8(kHi + 3 * j + k), sc, org);
+                } else {
+                    c[j].lo[k] = fmaf((float)r.u16(kLo + 2 * (2 * j + k)), sc, org);
+                    c[j].hi[k] = fmaf((float)r.u16(kHi + 2 * (2 * j + k)), sc, org);
+                }
+            }
+        }
+        constexpr float kS = 1.0f / 32767.0f;
+        const uint32_t rb = r.u32(kRef);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            float a[3] = {0.0f, 0.0f, 0.0f}, l = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                a[k] = r.s16(kAx + 2 * (D * j + k)) * kS;
+                l = fmaf(a[k], a[k], l);
+            }
+            const float rn = l > 0.0f ? 1.0f / sqrtf(l) : 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) c[j].axis[k] = a[k] * rn;
+            if constexpr (D == 2) c[j].lo[2] = c[j].hi[2] = 0.0f;
+            // the cosine is formed from the rounded-up sine, so it is no larger than the cosine of
+            // the widened angle (its float32 rounding sits inside the exact cull's margin)
+            const int qs = r.s16(kSin + 2 * j);
+            const float sn = qs * kS;
+            c[j].sin_t = sn;
+            c[j].cos_t = (flags & (kPairNone0 << j)) ? 2.0f
+                       : qs >= 32767             ? 0.0f   // never culls
+                                                 : sqrtf(fmaxf(fmaf(-sn, sn, 1.0f), 0.0f));
+            c[j].child = r.u32(kChild + 4 * j);
+            c[j].ref_cone = (flags & (kPairRef0 << j)) ? (int32_t)(rb + (j == 1 ? (flags & kPairRef0) : 0u)) : -1;
+            // extents in units of the decoded box's half-diagonal (scaled in exact_cull)
+            const float mid = r.s16(kExt + 2 * j) * kS, hw = r.u8(kExt + 4 + j) * (1.0f / 255.0f);
+            c[j].sil_lo = mid - hw * hw;
+            c[j].sil_hi = mid + hw * hw;
+            c[j].sil_perp = r.u8(kExt + 6 + j) * (1.0f / 255.0f);
+        }
+    } else if constexpr (D == 2) {
+        // SPairF<2> (192 B): lo[2][2] @0, hi[2][2] @16, cone axis @32, half @48,
         // radius @56, child @64, ref_cone @72, cull @80, sin(half) @88, cos(half) @96, exact-cull
         // cos @104, sin @112, axis @120, vertex extents along @136 / @144, across @152
         Words<40> r;
@@ -1050,42 +1105,6 @@ __device__ __forceinline__ void load_pair(const void *base, uint32_t idx, PairCh
             c[j].sil_lo = r.f32(136 + 4 * j);
             c[j].sil_hi = r.f32(144 + 4 * j);
             c[j].sil_perp = r.f32(152 + 4 * j);
-        }
-    } else if constexpr (Q) {
-        // SPairQ<3> (64 B): origin, exp | child boxes | cone_axis int16 @28 | cone_cs int16 @40 |
-        // child @48 | ref_cone @56
-        Words<16> r;
-        r.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64);
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const float org = r.f32(4 * k);
-            const float sc = ldexpf(1.0f, r.s8(4 * D + k));
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                if constexpr (D == 3) {
-                    c[j].lo[k] = fmaf((float)r.u8(16 + 3 * j + k), sc, org);
-                    c[j].hi[k] = fmaf((float)r.u8(22 + 3 * j + k), sc, org);
-                } else {
-                    c[j].lo[k] = fmaf((float)r.u16(12 + 2 * (2 * j + k)), sc, org);
-                    c[j].hi[k] = fmaf((float)r.u16(20 + 2 * (2 * j + k)), sc, org);
-                }
-            }
-        }
-        constexpr float kS = 1.0f / 32767.0f;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const float a0 = r.s16(28 + 6 * j) * kS, a1 = r.s16(30 + 6 * j) * kS, a2 = r.s16(32 + 6 * j) * kS;
-            const float l = a0 * a0 + a1 * a1 + a2 * a2;
-            const float rn = l > 0.0f ? 1.0f / sqrtf(l) : 0.0f;
-            c[j].axis[0] = a0 * rn;
-            c[j].axis[1] = a1 * rn;
-            c[j].axis[2] = a2 * rn;
-            const int cs0 = r.s16(40 + 4 * j), cs1 = r.s16(42 + 4 * j);
-            c[j].cos_t = (cs0 == 32767 && cs1 == -32767) ? 2.0f : cs0 * kS;
-            c[j].sin_t = cs1 * kS;
-            c[j].child = r.u32(48 + 4 * j);
-            c[j].ref_cone = (int32_t)r.u32(56 + 4 * j);
-            c[j].sil_lo = c[j].sil_hi = c[j].sil_perp = 0.0f;
         }
     } else {
         // SPairF<D> (128 B): lo[2][D] @0, hi[2][D] @8D, cones (axis3, cos, sin) @16D, child @16D+40,
@@ -1121,51 +1140,52 @@ template <int D, bool Q>
 __device__ __forceinline__ bool exact_cull(const PairChild &c, const float *xc, float mabs) {
     if (c.cos_t > 1.5f) return true;     // no silhouette can ever pass in this subtree
     if (!(c.cos_t > 0.0f)) return false; // cone too wide to separate anything
+    // a, b: the box centre relative to x along the cone axis u and across it
+    float a, b, hb2 = 0.0f;
     if constexpr (D == 3) {
-        float w[3], h[3];
+        float w[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             w[k] = 0.5f * (c.lo[k] + c.hi[k]) - xc[k];
-            h[k] = 0.5f * (c.hi[k] - c.lo[k]);
+            const float h = 0.5f * (c.hi[k] - c.lo[k]);
+            hb2 = fmaf(h, h, hb2);
         }
-        const float a = w[0] * c.axis[0] + w[1] * c.axis[1] + w[2] * c.axis[2];
+        a = w[0] * c.axis[0] + w[1] * c.axis[1] + w[2] * c.axis[2];
         const float cx = w[1] * c.axis[2] - w[2] * c.axis[1];
         const float cy = w[2] * c.axis[0] - w[0] * c.axis[2];
         const float cz = w[0] * c.axis[1] - w[1] * c.axis[0];
-        const float b = sqrtf(cx * cx + cy * cy + cz * cz);
-        if constexpr (Q) {
-            // quantized records: a sphere of the box's half-diagonal R around the centre bounds
-            // every edge point p, so dot(p - x, n) >= a cos - b sin - R for every cone normal n
-            const float R = sqrtf(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
-            const float m = R + 1e-5f * (fabsf(a) + b + R) + mabs;
-            return (a * c.cos_t - b * c.sin_t > m) || (a * c.cos_t + b * c.sin_t < -m);
-        } else {
-            // float32 records: the edges' extent along the axis, [lo_u, hi_u], and across it, rho.
-            // With n = cos(phi) u + sin(phi) t (phi <= theta, t _|_ u), dot(p - x, n) lies in
-            // [cos (a + lo_u) - sin (b + rho), cos (a + hi_u) + sin (b + rho)] at phi = theta, and
-            // the sphere through the extents (radius sqrt(max(lo_u^2, hi_u^2) + rho^2)) gives the
-            // bound above; either proof culls
-            const float lo_u = c.sil_lo, hi_u = c.sil_hi, rho = c.sil_perp;
-            const float R = sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho);
-            const float m = 1e-5f * (fabsf(a) + b + R) + mabs;
-            const bool sphere = (a * c.cos_t - b * c.sin_t > R + m) || (a * c.cos_t + b * c.sin_t < -(R + m));
-            const bool slab = ((a + lo_u) * c.cos_t - (b + rho) * c.sin_t > m) ||
-                              ((a + hi_u) * c.cos_t + (b + rho) * c.sin_t < -m);
-            return sphere || slab;
-        }
+        b = sqrtf(cx * cx + cy * cy + cz * cz);
     } else {
-        // 2D: the same bounds with the vertex extents (every record of this form is float32)
         const float w0 = 0.5f * (c.lo[0] + c.hi[0]) - xc[0], w1 = 0.5f * (c.lo[1] + c.hi[1]) - xc[1];
-        const float a = w0 * c.axis[0] + w1 * c.axis[1];
-        const float b = fabsf(w0 * c.axis[1] - w1 * c.axis[0]);
-        const float lo_u = c.sil_lo, hi_u = c.sil_hi, rho = c.sil_perp;
-        const float R = sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho);
-        const float m = 1e-5f * (fabsf(a) + b + R) + mabs;
-        const bool sphere = (a * c.cos_t - b * c.sin_t > R + m) || (a * c.cos_t + b * c.sin_t < -(R + m));
-        const bool slab = ((a + lo_u) * c.cos_t - (b + rho) * c.sin_t > m) ||
-                          ((a + hi_u) * c.cos_t + (b + rho) * c.sin_t < -m);
-        return sphere || slab;
+        const float h0 = 0.5f * (c.hi[0] - c.lo[0]), h1 = 0.5f * (c.hi[1] - c.lo[1]);
+        hb2 = fmaf(h0, h0, h1 * h1);
+        a = w0 * c.axis[0] + w1 * c.axis[1];
+        b = fabsf(w0 * c.axis[1] - w1 * c.axis[0]);
     }
+    // Every silhouette point p below lies, relative to the centre, within [lo_u, hi_u] along u and
+    // within rho of the line through the centre along u.  With n = cos(phi) u + sin(phi) t
+    // (phi <= theta, t _|_ u), dot(p - x, n) lies in
+    // [cos (a + lo_u) - sin (b + rho), cos (a + hi_u) + sin (b + rho)] at phi = theta; a sphere of
+    // radius R around the centre holding every point bounds it by
+    // [a cos - b sin - R, a cos + b sin + R].  Either proof culls.
+    float lo_u = c.sil_lo, hi_u = c.sil_hi, rho = c.sil_perp;
+    float R;
+    if constexpr (Q) {
+        // quantized records: the extents are in units of the decoded box's half-diagonal, and the
+        // sphere of that radius around the centre holds the whole box
+        const float Rb = sqrtf(hb2);
+        lo_u *= Rb;
+        hi_u *= Rb;
+        rho *= Rb;
+        R = fminf(Rb, sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho));
+    } else {
+        R = sqrtf(fmaxf(lo_u * lo_u, hi_u * hi_u) + rho * rho);
+    }
+    const float m = 1e-5f * (fabsf(a) + b + R) + mabs;
+    const bool sphere = (a * c.cos_t - b * c.sin_t > R + m) || (a * c.cos_t + b * c.sin_t < -(R + m));
+    const bool slab = ((a + lo_u) * c.cos_t - (b + rho) * c.sin_t > m) ||
+                      ((a + hi_u) * c.cos_t + (b + rho) * c.sin_t < -m);
+    return sphere || slab;
 }
 
 
@@ -1206,11 +1226,24 @@ __device__ __forceinline__ bool ref_cone_2d(const PairChild &c, V2 x, float r, f
     return snch_overlaps(lo, hi, v2(c.ref_axis[0], c.ref_axis[1]), c.half, c.radius, x, r, d_min);
 }
 
-template <int D>
+template <int D, bool Q>
 __device__ __forceinline__ bool ref_cone_pass(const GP &P, const PairChild &c, const typename VT<D>::T &x, float r,
                                               float &d_min) {
     if constexpr (D == 2) {
-        return ref_cone_2d(c, x, r, d_min);
+        if constexpr (Q) {
+            // quantized 2D records keep each child's reference box and cone in a table (RefCone2:
+            // lo, hi | axis, half, radius | sin, cos of the half-angle), fetched only for a child
+            // that passed the box test and the exact cull
+            const GQ_G float4 *e = (const GQ_G float4 *)P.ref_cones + 3 * c.ref_cone;
+            const float4 e0 = ldg(e), e1 = ldg(e + 1), e2 = ldg(e + 2);
+            PairChild t = c;
+            t.lo[0] = e0.x; t.lo[1] = e0.y; t.hi[0] = e0.z; t.hi[1] = e0.w;
+            t.ref_axis[0] = e1.x; t.ref_axis[1] = e1.y; t.half = e1.z; t.radius = e1.w;
+            t.sin_h = e2.x; t.cos_h = e2.y;
+            return ref_cone_2d(t, x, r, d_min);
+        } else {
+            return ref_cone_2d(c, x, r, d_min);
+        }
     } else {
         const RefCone rc = ldg(P.ref_cones + c.ref_cone);
         return snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, d_min);
@@ -1271,8 +1304,9 @@ __device__ __forceinline__ void sil_packet(const GP &P, int base) {
                     float dm, dx;
                     box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                     bool h = act && dm <= r;
-                    if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
+                    // the exact cull first: it needs no fetch, the reference cone test a table entry
                     if (h && exact_cull<D, Q>(c[j], xc, cslack)) h = false;
+                    if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D, Q>(P, c[j], x, r, dm);
                     hh[j] = h;
                     hm[j] = __ballot(h);
                     any[j] = hm[j] != 0ull;
@@ -1426,11 +1460,11 @@ __device__ __forceinline__ void sil_lane(const GP &P, int base) {
                 float dm, dx;
                 box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                 bool hj = dm <= r;
+                if (hj && exact_cull<D, Q>(c[j], xc, cslack)) hj = false;
                 if (hj && c[j].ref_cone >= 0) {
                     float dr;
-                    hj = ref_cone_pass<D>(P, c[j], x, r, dr);
+                    hj = ref_cone_pass<D, Q>(P, c[j], x, r, dr);
                 }
-                if (hj && exact_cull<D, Q>(c[j], xc, cslack)) hj = false;
                 h[j] = hj;
                 key[j] = dm;
             }
@@ -1525,8 +1559,8 @@ __device__ __forceinline__ void sil_wave_item(const GP &P, const uint4 &item, ui
                         float dm, dx;
                         box_distance(ldv<D>(c[j].lo), ldv<D>(c[j].hi), x, dm, dx);
                         bool h = dm <= r;
-                        if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D>(P, c[j], x, r, dm);
                         if (h && exact_cull<D, Q>(c[j], xc, cslack)) h = false;
+                        if (h && c[j].ref_cone >= 0) h = ref_cone_pass<D, Q>(P, c[j], x, r, dm);
                         const uint2 ent = make_uint2(c[j].child, __float_as_uint(dm));
                         if (j == 0) { h0 = h; ch0 = ent; }
                         else { h1 = h; ch1 = ent; }
@@ -2036,7 +2070,8 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             P.nodes = jb.quantized ? w.qsnodes : w.snodes;
             P.prims = w.sils;
             P.ids = w.sil_ids;
-            P.ref_cones = w.ref_cones;
+            // 2D: the quantized pair records' table (RefCone2); the float32 ones carry theirs inline
+            P.ref_cones = w.dim == 2 ? (const RefCone *)w.ref_cones2 : w.ref_cones;
             P.root = w.sroot_ref;
             for (int k = 0; k < 3; ++k) { P.root_lo[k] = w.sroot_lo[k]; P.root_hi[k] = w.sroot_hi[k]; }
             P.seed = (const Seed *)w.sil_seed;

@@ -330,10 +330,11 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     if (h.is_snch) {
         if ((rc = upload(s.get(), wh.snodes.data(), wh.snodes.size(), (const uint8_t **)&wv.snodes))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.qsnodes.data(), wh.qsnodes.size(), (const uint8_t **)&wv.qsnodes))) return cleanup(rc);
-        if (h.dim == 2) wv.qsnodes = wv.snodes;   // 2D pairs have one (float32, cones inline) format
         if ((rc = upload(s.get(), wh.sils.data(), wh.sils.size(), &wv.sils))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.sil_ids.data(), wh.sil_ids.size(), &wv.sil_ids))) return cleanup(rc);
         if ((rc = upload(s.get(), wh.ref_cones.data(), wh.ref_cones.size(), &wv.ref_cones))) return cleanup(rc);
+        if ((rc = upload(s.get(), wh.ref_cones2.data(), wh.ref_cones2.size(), (const RefCone2 **)&wv.ref_cones2)))
+            return cleanup(rc);
     }
     *out = s.release();
     return GQ_OK;
@@ -352,13 +353,14 @@ int gq_scene_info(const gq_scene *s, int64_t info[8]) {
     return GQ_OK;
 }
 
-// Whether a query runs on the quantized records.  3D silhouettes use the float32 pair records
-// unless the scene asks for the 8-bit ones: their exact boxes and stored edge extents open fewer
-// nodes (16.8 vs 26.3 ms on 1M queries against 327,680 triangles), and that pass is bound by node
-// tests, not by bytes.
+// Whether a query runs on the quantized records.  Silhouettes use the float32 pair records unless
+// the scene asks for the quantized ones: exact boxes open fewer nodes, and the 2D record carries
+// the reference cones inline where the quantized one fetches them from a table (1M queries: 14.9
+// vs 18.4 ms against 327,680 triangles, 0.92 vs 1.54 ms against 65,536 segments); that pass is
+// bound by node tests and fetch latency, not by bytes.
 static bool quantized_layout(int method, QueryKind kind, const WideView &w) {
     if (method != GQ_METHOD_ENGINE) return false;
-    return !(kind == kSilhouette && w.dim == 3 && w.sil_pairs == 0);
+    return !(kind == kSilhouette && w.sil_pairs == 0);
 }
 
 static int run_query(gq_scene *s, QueryKind kind, int method, const QueryArgs &q, void *stream) {
@@ -525,6 +527,9 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     case 8: src = wh.ref_cones.data(); bytes = wh.ref_cones.size() * sizeof(RefCone); break;
     case 9: src = wh.big_leaves.data(); bytes = wh.big_leaves.size() * 4; break;
     case 10: src = scal; bytes = sizeof scal; break;
+    case 11: src = wh.ref_cones2.data(); bytes = wh.ref_cones2.size() * sizeof(RefCone2); break;
+    case 12: src = wh.bvh_seed.data(); bytes = wh.bvh_seed.size() * sizeof(Seed); break;
+    case 13: src = wh.sil_seed.data(); bytes = wh.sil_seed.size() * sizeof(Seed); break;
     default: return fail(GQ_ERR_INVALID, "unknown layout array");
     }
     if (out) {

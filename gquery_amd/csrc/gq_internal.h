@@ -41,6 +41,7 @@ struct WideView {
     const float4 *sils;       // 3D: 3 float4 per silhouette (a, b, n0, n1); 2D: 2 float4 (a|b, c)
     const int32_t *sil_ids;
     const RefCone *ref_cones;
+    const void *ref_cones2;   // RefCone2[]: 2D reference cones of the quantized pair records
     const uint32_t *big_leaves;
     const void *bvh_seed, *sil_seed;   // Seed[n_bvh_seed], Seed[n_sil_seed]
     int n_bvh_seed, n_sil_seed;
@@ -50,7 +51,7 @@ struct WideView {
     int packet_budget;        // 0 = default, < 0 = no deferral to the wave-per-query pass
     int traversal;            // silhouettes: 0 auto, 1 packet + wave pass, 2 lane per query
     int wave_front;           // first wave-pass frontier entries (0 = default)
-    int sil_pairs;            // 3D silhouettes under the quantized method: 0 float32 pairs, 1 8-bit
+    int sil_pairs;            // silhouettes under the quantized method: 0 float32 pairs, 1 quantized
 };
 
 struct QueryArgs {

@@ -208,6 +208,99 @@ struct SilNormals {
     bool always;  // boundary edge: a silhouette for every query point
 };
 
+// Quantized pair record from the float32 children (boxes, exact-cull cones, child refs).
+// `points(j, f)` calls f(x, y, z) for every silhouette point below child j (3D: both edge
+// endpoints; 2D: the vertex, z = 0).  Reference-cone flags and ref_base are the caller's.
+template <int D, typename Points>
+void make_qpair(const float (&lo)[2][D], const float (&hi)[2][D], const ConeF (&cone)[2], const uint32_t (&child)[2],
+                Points points, SPairQ<D> &pq) {
+    constexpr int L = QBits<D>::levels;
+    int e[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) {
+        const float flo = std::min(lo[0][k], lo[1][k]), fhi = std::max(hi[0][k], hi[1][k]);
+        e[k] = std::clamp(frame_exp(flo, fhi, L), -127, 127);
+        pq.origin[k] = flo;
+        pq.exp[k] = (int8_t)e[k];
+    }
+    for (int j = 0; j < 2; ++j) {
+        float dlo[3] = {0, 0, 0}, dhi[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) {
+            int ql, qh;
+            quantize(lo[j][k], hi[j][k], pq.origin[k], e[k], L, ql, qh);
+            pq.qlo[j][k] = (typename QBits<D>::T)ql;
+            pq.qhi[j][k] = (typename QBits<D>::T)qh;
+            dlo[k] = decode(ql, e[k], pq.origin[k]);
+            dhi[k] = decode(qh, e[k], pq.origin[k]);
+        }
+        pq.child[j] = child[j];
+        // int16 cone: the axis rounding error is folded into the angle
+        const ConeF &cf = cone[j];
+        int16_t qa[3] = {0, 0, 0};
+        double dq[3] = {0, 0, 0}, fa[3] = {0, 0, 0}, lq2 = 0.0, lf2 = 0.0;
+        for (int k = 0; k < D; ++k) {
+            qa[k] = (int16_t)std::lround(std::clamp((double)cf.axis[k], -1.0, 1.0) * 32767.0);
+            pq.cone_axis[j][k] = qa[k];
+            dq[k] = qa[k] / 32767.0;
+            fa[k] = cf.axis[k];
+            lq2 += dq[k] * dq[k];
+            lf2 += fa[k] * fa[k];
+        }
+        const double lq = std::sqrt(lq2), lf = std::sqrt(lf2);
+        // angle between the float axis and the int16 one, as atan2(|a x b|, a . b) (acos of a
+        // dot product near 1 would turn the float axis's 1e-7 length error into 3e-4 rad)
+        double dot = 0.0, cr2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            dot += fa[k] * dq[k];
+            const double ck = fa[(k + 1) % 3] * dq[(k + 2) % 3] - fa[(k + 2) % 3] * dq[(k + 1) % 3];
+            cr2 += ck * ck;
+        }
+        pq.cone_sin[j] = 32767;   // never culls
+        if (cf.cos_t > 1.5f) {
+            pq.flags |= (uint8_t)(kPairNone0 << j);
+        } else if (cf.cos_t > 0.0f && lq > 0 && lf > 0) {
+            const double err = std::atan2(std::sqrt(cr2), dot) + 2.0 / 32767.0;
+            const double th = std::acos(std::clamp((double)cf.cos_t, -1.0, 1.0)) + err;
+            if (th < kCullMaxAngle) pq.cone_sin[j] = (int16_t)std::min(32767.0, std::ceil(std::sin(th) * 32767.0));
+        }
+        // extent of the points about the decoded box centre (as the kernels form it) along and
+        // across the decoded axis, in units of the decoded box's half-diagonal R
+        float cen[3] = {0, 0, 0};
+        double R2 = 0.0, u[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) {
+            cen[k] = 0.5f * (dlo[k] + dhi[k]);
+            const double hk = (double)(0.5f * (dhi[k] - dlo[k]));
+            R2 += hk * hk;
+            u[k] = lq > 0 ? dq[k] / lq : 0.0;
+        }
+        const double R = std::sqrt(R2);
+        double amin = 0.0, amax = 0.0, perp = 0.0;
+        bool any = false;
+        points(j, [&](double px, double py, double pz) {
+            const double d[3] = {px - cen[0], py - cen[1], D == 3 ? pz - cen[2] : 0.0};
+            const double al = d[0] * u[0] + d[1] * u[1] + d[2] * u[2];
+            double q2 = 0.0;
+            for (int k = 0; k < 3; ++k) q2 += (d[k] - al * u[k]) * (d[k] - al * u[k]);
+            amin = any ? std::min(amin, al) : al;
+            amax = any ? std::max(amax, al) : al;
+            perp = std::max(perp, std::sqrt(q2));
+            any = true;
+        });
+        int mid = 0, half = 255, sp = 255;
+        if (any && R > 0) {
+            mid = std::clamp((int)std::lround(0.5 * (amin + amax) / R * 32767.0), -32767, 32767);
+            const double m = mid / 32767.0 * R;
+            const double need = std::max(m - amin, amax - m) / R;   // half-width, in units of R
+            half = (int)std::ceil(std::sqrt(std::max(need, 0.0)) * 255.0 + 1e-3);
+            sp = (int)std::ceil(perp / R * 255.0 + 1e-3);
+        } else if (any) {
+            half = sp = 0;   // a point box: every point is the centre
+        }
+        pq.ext_mid[j] = (int16_t)mid;
+        pq.ext_half[j] = (uint8_t)std::clamp(half, 0, 255);
+        pq.ext_perp[j] = (uint8_t)std::clamp(sp, 0, 255);
+    }
+}
+
 template <int D>
 void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, WideHost &out, LeafTable &lt) {
     const int nss = 3 * D + 7, ss = kSilStride[D];
@@ -342,13 +435,15 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
     out.snodes.assign((size_t)std::max<uint32_t>(np, 1) * sizeof(SPairF<D>), 0);
     const float kHalfPi = 1.57079637050628662109375f;
     if constexpr (D == 2) {
-        // one float32 record per pair with the reference boxes and cones inline (no quantized twin)
-        out.qsnodes.clear();
+        // a float32 record per pair with the reference boxes and cones inline, and its quantized
+        // twin whose reference cones sit in a table
+        out.qsnodes.assign((size_t)std::max<uint32_t>(np, 1) * sizeof(SPairQ<2>), 0);
         for (size_t i = 0; i < nn; ++i) {
             if (b[i].leaf) continue;
             const size_t ch[2] = {i + 1, i + b[i].second};
             SPairF<2> pf;
             std::memset(&pf, 0, sizeof pf);
+            ConeF cfs[2];
             for (int j = 0; j < 2; ++j) {
                 const size_t c = ch[j];
                 const float *n = snd + c * nss;   // SNCH node: box 0..3, axis 4..5, half 6, radius 7
@@ -365,7 +460,7 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 // the reference cone test can only cull when 0 <= half <= pi/2 (a wider cone always
                 // contains a direction perpendicular to every view direction; a negative one is unset)
                 pf.ref_cone[j] = (n[6] >= 0.0f && !(n[6] > kHalfPi)) ? 0 : -1;
-                ConeF cf;
+                ConeF &cf = cfs[j];
                 cone_of(c, cf);
                 pf.cull[j] = cf.cos_t > 1.5f ? 1 : 0;
                 pf.ecos[j] = cf.cos_t;
@@ -393,10 +488,33 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 pf.eperp[j] = std::nextafter((float)perp, kInfF);
             }
             std::memcpy(out.snodes.data() + (size_t)pidx[i] * sizeof(SPairF<2>), &pf, sizeof pf);
+            SPairQ<2> pq;
+            std::memset(&pq, 0, sizeof pq);
+            make_qpair<2>(pf.lo, pf.hi, cfs, pf.child, [&](int j, auto f) {
+                const size_t c = ch[j];
+                for (uint32_t t = sb[c]; t < sb[c] + sc[c]; ++t) f(out.sils[2 * t].z, out.sils[2 * t].w, 0.0);
+            }, pq);
+            pq.ref_base = (int32_t)out.ref_cones2.size();
+            for (int j = 0; j < 2; ++j) {
+                if (pf.ref_cone[j] < 0) continue;
+                RefCone2 rc;
+                std::memset(&rc, 0, sizeof rc);
+                for (int k = 0; k < 2; ++k) {
+                    rc.lo[k] = pf.lo[j][k];
+                    rc.hi[k] = pf.hi[j][k];
+                    rc.axis[k] = pf.cone_axis[j][k];
+                }
+                rc.half = pf.cone_half[j];
+                rc.radius = pf.cone_radius[j];
+                rc.sin_h = pf.cone_sin[j];
+                rc.cos_h = pf.cone_cos[j];
+                pq.flags |= (uint8_t)(kPairRef0 << j);
+                out.ref_cones2.push_back(rc);
+            }
+            std::memcpy(out.qsnodes.data() + (size_t)pidx[i] * sizeof(SPairQ<2>), &pq, sizeof pq);
         }
     } else {
     out.qsnodes.assign((size_t)std::max<uint32_t>(np, 1) * sizeof(SPairQ<D>), 0);
-    constexpr int L = QBits<D>::levels;
     for (size_t i = 0; i < nn; ++i) {
         if (b[i].leaf) continue;
         const size_t ch[2] = {i + 1, i + b[i].second};
@@ -404,26 +522,15 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
         std::memset(&pf, 0, sizeof pf);
         SPairQ<D> pq;
         std::memset(&pq, 0, sizeof pq);
-        int e[3];
-        for (int k = 0; k < D; ++k) {
-            float lo = std::min(b[ch[0]].lo[k], b[ch[1]].lo[k]);
-            float hi = std::max(b[ch[0]].hi[k], b[ch[1]].hi[k]);
-            e[k] = std::clamp(frame_exp(lo, hi, L), -127, 127);
-            pq.origin[k] = lo;
-            pq.exp[k] = (int8_t)e[k];
-        }
+        pq.ref_base = (int32_t)out.ref_cones.size();
         for (int j = 0; j < 2; ++j) {
             const size_t c = ch[j];
             const BinNode &x = b[c];
             for (int k = 0; k < D; ++k) {
                 pf.lo[j][k] = x.lo[k];
                 pf.hi[j][k] = x.hi[k];
-                int ql, qh;
-                quantize(x.lo[k], x.hi[k], pq.origin[k], e[k], L, ql, qh);
-                pq.qlo[j][k] = (typename QBits<D>::T)ql;
-                pq.qhi[j][k] = (typename QBits<D>::T)qh;
             }
-            pf.child[j] = pq.child[j] = child_ref(c, pidx);
+            pf.child[j] = child_ref(c, pidx);
             // the reference cone test can only cull when 0 <= half <= pi/2 (a wider cone always
             // contains a direction perpendicular to every view direction; a negative one is unset)
             const float *n = snd + c * nss;
@@ -434,10 +541,11 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 for (int k = 0; k < D; ++k) { rc.axis[k] = n[2 * D + k]; rc.lo[k] = n[k]; rc.hi[k] = n[D + k]; }
                 rc.half = half;
                 rc.radius = n[3 * D + 1];
-                pf.ref_cone[j] = pq.ref_cone[j] = (int32_t)out.ref_cones.size();
+                pf.ref_cone[j] = (int32_t)out.ref_cones.size();
+                pq.flags |= (uint8_t)(kPairRef0 << j);
                 out.ref_cones.push_back(rc);
             } else {
-                pf.ref_cone[j] = pq.ref_cone[j] = -1;
+                pf.ref_cone[j] = -1;
             }
             ConeF cf;
             cone_of(c, cf);
@@ -446,7 +554,7 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 // extent of the silhouette edges below along and across the cone axis, about the
                 // centre the kernel forms (float32, 0.5f * (lo + hi)): the exact cull bounds every
                 // edge point by it (a slab across a nearly flat patch is far thinner than the sphere
-                // around it, which the quantized record falls back to)
+                // around it)
                 float cen[3];
                 for (int k = 0; k < 3; ++k) cen[k] = 0.5f * (pf.lo[j][k] + pf.hi[j][k]);
                 const double u[3] = {cf.axis[0], cf.axis[1], cf.axis[2]};
@@ -471,30 +579,15 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 pf.sil_hi[j] = std::nextafter((float)amax, kInfF);
                 pf.sil_perp[j] = std::nextafter((float)std::sqrt(perp2), kInfF);
             }
-            // int16 cone: the axis rounding error is folded into the angle
-            double ax[3] = {cf.axis[0], cf.axis[1], cf.axis[2]};
-            int16_t qa[3];
-            for (int k = 0; k < 3; ++k) qa[k] = (int16_t)std::lround(std::clamp(ax[k], -1.0, 1.0) * 32767.0);
-            double dq[3] = {qa[0] / 32767.0, qa[1] / 32767.0, qa[2] / 32767.0};
-            double lq = std::sqrt(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
-            for (int k = 0; k < 3; ++k) pq.cone_axis[j][k] = qa[k];
-            if (cf.cos_t > 1.5f) {
-                pq.cone_cs[j][0] = 32767;   // "always cull" (no silhouettes below)
-                pq.cone_cs[j][1] = -32767;
-            } else if (cf.cos_t <= 0.0f || !(lq > 0)) {
-                pq.cone_cs[j][0] = 0;
-                pq.cone_cs[j][1] = 32767;
-            } else {
-                double dot = (ax[0] * dq[0] + ax[1] * dq[1] + ax[2] * dq[2]) / lq;
-                double err = std::acos(std::clamp(dot, -1.0, 1.0)) + 2.0 / 32767.0;
-                double th = std::acos(std::clamp((double)cf.cos_t, -1.0, 1.0)) + err;
-                if (th >= kCullMaxAngle) { pq.cone_cs[j][0] = 0; pq.cone_cs[j][1] = 32767; }
-                else {
-                    pq.cone_cs[j][0] = (int16_t)std::floor(std::cos(th) * 32767.0);
-                    pq.cone_cs[j][1] = (int16_t)std::min(32767.0, std::ceil(std::sin(th) * 32767.0));
-                }
-            }
         }
+        make_qpair<D>(pf.lo, pf.hi, pf.cone, pf.child, [&](int j, auto f) {
+            const size_t c = ch[j];
+            for (uint32_t t = sb[c]; t < sb[c] + sc[c]; ++t) {
+                const float4 s0 = out.sils[3 * t], s1 = out.sils[3 * t + 1];
+                f(s0.x, s0.y, s0.z);
+                f(s0.w, s1.x, s1.y);
+            }
+        }, pq);
         std::memcpy(out.snodes.data() + (size_t)pidx[i] * sizeof(SPairF<D>), &pf, sizeof pf);
         std::memcpy(out.qsnodes.data() + (size_t)pidx[i] * sizeof(SPairQ<D>), &pq, sizeof pq);
     }

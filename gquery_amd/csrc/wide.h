@@ -8,7 +8,9 @@
 //      FNode<D> 128 B: child boxes in float32.
 //  * SNCH pairs: one record per binary interior node holding both children (box, reference
 //    normal-cone index, exact-cull cone, child reference) so a visit is one record fetch.
-//      SPairQ<3> 64 B (quantized boxes / cones), SPairF<D> 128 B in 3D / 192 B in 2D (float32).
+//      SPairQ<D> 64 B (boxes quantized to 8 bits in 3D / 16 bits in 2D, int16 cones, 8-bit
+//      silhouette extents, reference cones by table index), SPairF<D> 128 B in 3D / 192 B in 2D
+//      (float32; the 2D record carries its reference cones inline).
 //  * primitives in leaf order as float4 records (3D: a, b, c; 2D: a|b), silhouettes in leaf
 //    order (3D: a, b, n0, n1 with the predicate normals precomputed; 2D: a|b, c).
 // Child references: bit 31 set = leaf, bits 24..30 = count, bits 0..23 = first primitive /
@@ -119,25 +121,54 @@ template <> struct alignas(64) SPairF<2> {
 };
 static_assert(sizeof(SPairF<3>) == 128 && sizeof(SPairF<2>) == 192, "SPairF: two lines in 3D, three in 2D");
 
-// SNCH pair record (3D), 8-bit quantized boxes and int16 cones.
+// SNCH pair record, quantized: child boxes in 8 bits (3D) / 16 bits (2D) relative to the pair's
+// frame, rounded outward; exact-cull cones as an int16 axis (its rounding folded into the angle)
+// and the int16 sine of the widened angle, rounded up (32767: never culls; the kernels form the
+// cosine from it, which keeps narrow cones narrow); the silhouette points' extent about the decoded
+// box centre, in units of R, the decoded box's half-diagonal: along the decoded axis a slab of
+// centre ext_mid * R / 32767 (rounded to nearest) and half-width (ext_half / 255)^2 * R (rounded
+// up, fine near zero, where thin slabs sit), across the axis ext_perp * R / 255 (rounded up);
+// reference cones in a table (3D: RefCone, 2D: RefCone2), child j's at
+// ref_base + (j == 1 && child 0 has one).
+constexpr uint8_t kPairRef0 = 1, kPairRef1 = 2, kPairNone0 = 4, kPairNone1 = 8;   // SPairQ flags
 template <int D> struct SPairQ;
 template <> struct alignas(64) SPairQ<3> {
     float origin[3];
     int8_t exp[3];
-    uint8_t pad0;
+    uint8_t flags;             // kPairRef<j>: child j has a reference cone; kPairNone<j>: no silhouette below
     uint8_t qlo[2][3], qhi[2][3];
-    int16_t cone_axis[2][3];   // unit axis * 32767 (rounded; the error is added to the angle)
-    int16_t cone_cs[2][2];     // cos_t (rounded down), sin_t (rounded up), * 32767
+    int16_t cone_axis[2][3];
+    int16_t cone_sin[2];
     uint32_t child[2];
-    int32_t ref_cone[2];
+    int32_t ref_base;
+    int16_t ext_mid[2];
+    uint8_t ext_half[2], ext_perp[2];
 };
-static_assert(sizeof(SPairQ<3>) == 64, "SPairQ is one line");
+template <> struct alignas(64) SPairQ<2> {
+    float origin[2];
+    int8_t exp[2];
+    uint8_t flags, pad0;
+    uint16_t qlo[2][2], qhi[2][2];
+    int16_t cone_axis[2][2];
+    int16_t cone_sin[2];
+    uint32_t child[2];
+    int32_t ref_base;
+    int16_t ext_mid[2];
+    uint8_t ext_half[2], ext_perp[2];
+    uint8_t pad[4];
+};
+static_assert(sizeof(SPairQ<3>) == 64 && sizeof(SPairQ<2>) == 64, "SPairQ is one line");
 // byte offsets the kernels decode (engine.hip load_pair / load_node4)
-static_assert(offsetof(SPairQ<3>, qlo) == 16 && offsetof(SPairQ<3>, qhi) == 22, "SPairQ<3> boxes");
-static_assert(offsetof(SPairQ<3>, cone_axis) == 28, "cone axis");
-static_assert(offsetof(SPairQ<3>, cone_cs) == 40, "cone cs");
-static_assert(offsetof(SPairQ<3>, child) == 48, "pair child");
-static_assert(offsetof(SPairQ<3>, ref_cone) == 56, "pair ref");
+static_assert(offsetof(SPairQ<3>, flags) == 15 && offsetof(SPairQ<3>, qlo) == 16 && offsetof(SPairQ<3>, qhi) == 22 &&
+              offsetof(SPairQ<3>, cone_axis) == 28 && offsetof(SPairQ<3>, cone_sin) == 40 &&
+              offsetof(SPairQ<3>, child) == 44 && offsetof(SPairQ<3>, ref_base) == 52 &&
+              offsetof(SPairQ<3>, ext_mid) == 56 && offsetof(SPairQ<3>, ext_half) == 60 &&
+              offsetof(SPairQ<3>, ext_perp) == 62, "SPairQ<3>");
+static_assert(offsetof(SPairQ<2>, flags) == 10 && offsetof(SPairQ<2>, qlo) == 12 && offsetof(SPairQ<2>, qhi) == 20 &&
+              offsetof(SPairQ<2>, cone_axis) == 28 && offsetof(SPairQ<2>, cone_sin) == 36 &&
+              offsetof(SPairQ<2>, child) == 40 && offsetof(SPairQ<2>, ref_base) == 48 &&
+              offsetof(SPairQ<2>, ext_mid) == 52 && offsetof(SPairQ<2>, ext_half) == 56 &&
+              offsetof(SPairQ<2>, ext_perp) == 58, "SPairQ<2>");
 static_assert(offsetof(SPairF<3>, cone) == 48 && offsetof(SPairF<3>, child) == 88 && offsetof(SPairF<3>, ref_cone) == 96 &&
               offsetof(SPairF<3>, sil_lo) == 104 && offsetof(SPairF<3>, sil_perp) == 120, "SPairF<3>");
 static_assert(offsetof(SPairF<2>, cone_axis) == 32 && offsetof(SPairF<2>, cone_half) == 48 &&
@@ -160,6 +191,16 @@ struct RefCone {
     float half, radius;
     float lo[3], hi[3];   // the node's box (the reference test uses its centroid and p_max)
 };
+// 2D reference cone of a child of a quantized pair record: the float32 box and cone the 2D pair
+// record carries inline (three float4).
+struct RefCone2 {
+    float lo[2], hi[2];
+    float axis[2];
+    float half, radius;
+    float sin_h, cos_h;   // sin and cos of the half-angle
+    float pad[2];
+};
+static_assert(sizeof(RefCone2) == 48, "RefCone2 is three float4");
 
 // One entry of a seed cut: a node (child ref) below the root and its float32 box.  The
 // wave-per-query passes start from the cut (the root's descendants at a fixed depth, as far as the
@@ -187,6 +228,7 @@ struct WideHost {
     std::vector<float4> sils;
     std::vector<int32_t> sil_ids;
     std::vector<RefCone> ref_cones;
+    std::vector<RefCone2> ref_cones2;
     std::vector<uint32_t> big_leaves;  // (first, count) pairs of leaves with more than 126 entries
     std::vector<Seed> bvh_seed, sil_seed;   // seed cuts of the BVH4 and of the SNCH pairs
 };

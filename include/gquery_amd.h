@@ -38,7 +38,7 @@ extern "C" {
 /* Traversal method of a query call. */
 #define GQ_METHOD_PLAIN 0       /* reference layout: binary AoS float32 nodes, lane per query */
 #define GQ_METHOD_ENGINE_FP32 1 /* wide SoA nodes with float32 bounds, sorted queries */
-#define GQ_METHOD_ENGINE 2      /* wide nodes with 8-bit (3D) / 16-bit (2D) quantized bounds; 3D
+#define GQ_METHOD_ENGINE 2      /* wide nodes with 8-bit (3D) / 16-bit (2D) quantized bounds;
                                    silhouettes on float32 pairs unless sil_pairs = 1 (below) */
 
 typedef struct gq_structure gq_structure; /* host-side BVH or SNCH (CPU builder output) */
@@ -129,10 +129,10 @@ typedef struct gq_scene_options {
                               0 = default (closest point 1024, silhouettes 640).  A query whose
                               frontier outgrows it is finished by a second pass with the full
                               frontier: results do not change */
-    int32_t sil_pairs;     /* 3D silhouette pair records under GQ_METHOD_ENGINE: 0 = float32 (128 B
-                              per pair, default: exact boxes and the edges' extent for the exact
-                              cull, fewer nodes opened), 1 = 8-bit quantized (64 B per pair, half
-                              the memory).  Results do not change */
+    int32_t sil_pairs;     /* silhouette pair records under GQ_METHOD_ENGINE: 0 = float32 (default;
+                              128 B per pair in 3D, 192 B in 2D with the reference cones inline),
+                              1 = quantized (64 B per pair: 8-bit boxes in 3D, 16-bit in 2D, int16
+                              cones, 2D reference cones from a table).  Results do not change */
     int32_t reserved[3];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
@@ -260,11 +260,13 @@ int gq_configure_silhouette(const float *vertices, int64_t n_vertices, const int
 /* ---- Engine layout inspection (no counterpart in the reference: exposes the host-built engine
  * arrays so tests can check them on machines without a GPU) ---- */
 
-/* which: 0 quantized BVH4 nodes, 1 float32 BVH4 nodes, 2 quantized SNCH pairs (3D only), 3 float32
- * SNCH pairs (2D: with the reference cones inline), 4 primitives (float4), 5 primitive ids,
- * 6 silhouettes (float4), 7 silhouette ids, 8 reference cones (3D only), 9 big-leaf table,
- * 10 scalars (root_ref, sroot_ref, n_nodes, n_pairs, wide_depth as uint32).  Pass out = NULL to
- * get the byte count in *n_bytes. */
+/* which: 0 quantized BVH4 nodes, 1 float32 BVH4 nodes, 2 quantized SNCH pairs (8-bit boxes in 3D,
+ * 16-bit in 2D), 3 float32 SNCH pairs (2D: with the reference cones inline), 4 primitives
+ * (float4), 5 primitive ids, 6 silhouettes (float4), 7 silhouette ids, 8 reference cones (3D),
+ * 9 big-leaf table, 10 scalars (root_ref, sroot_ref, n_nodes, n_pairs, wide_depth as uint32),
+ * 11 reference cones of the 2D quantized pairs, 12 seed cut of the BVH4, 13 seed cut of the SNCH
+ * pairs (32-byte entries: box low corner, child reference, box high corner, pad).  Pass
+ * out = NULL to get the byte count in *n_bytes. */
 int gq_structure_engine_layout(const gq_structure *s, int which, void *out, int64_t *n_bytes);
 
 /* ---- Kernel timing (replaces the time.time() brackets around dr.eval in the reference demos,

@@ -23,10 +23,21 @@ def fnode_dtype(D):
 
 
 def spq_dtype(D):
-    assert D == 3   # 2D pairs have only the float32 record
-    return np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("pad0", "u1"), ("qlo", "u1", (2, 3)),
-                     ("qhi", "u1", (2, 3)), ("axis", "<i2", (2, 3)), ("cs", "<i2", (2, 2)),
-                     ("child", "<u4", 2), ("ref", "<i4", 2)])
+    if D == 3:
+        return np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("flags", "u1"), ("qlo", "u1", (2, 3)),
+                         ("qhi", "u1", (2, 3)), ("axis", "<i2", (2, 3)), ("sin", "<i2", 2),
+                         ("child", "<u4", 2), ("ref_base", "<i4"), ("ext_mid", "<i2", 2), ("ext_half", "u1", 2),
+                         ("ext_perp", "u1", 2)])
+    return np.dtype([("origin", "<f4", 2), ("exp", "i1", 2), ("flags", "u1"), ("pad0", "u1"),
+                     ("qlo", "<u2", (2, 2)), ("qhi", "<u2", (2, 2)), ("axis", "<i2", (2, 2)), ("sin", "<i2", 2),
+                     ("child", "<u4", 2), ("ref_base", "<i4"), ("ext_mid", "<i2", 2), ("ext_half", "u1", 2),
+                     ("ext_perp", "u1", 2), ("pad", "u1", 4)])
+
+
+REFCONE = np.dtype([("axis", "<f4", 3), ("half", "<f4"), ("radius", "<f4"), ("lo", "<f4", 3), ("hi", "<f4", 3)])
+REFCONE2 = np.dtype([("lo", "<f4", 2), ("hi", "<f4", 2), ("axis", "<f4", 2), ("half", "<f4"), ("radius", "<f4"),
+                     ("sin", "<f4"), ("cos", "<f4"), ("pad", "<f4", 2)])
+SEED = np.dtype([("lo", "<f4", 3), ("ref", "<u4"), ("hi", "<f4", 3), ("pad", "<f4")])
 
 
 def spf_dtype(D):
@@ -139,11 +150,10 @@ def test_snch_pair_layout(name):
     scal = s.engine_layout(10).view(np.uint32)
     sroot, n_pairs = scal[1], scal[3]
     pf = s.engine_layout(3).view(spf_dtype(D))
-    if D == 3:
-        pq = s.engine_layout(2).view(spq_dtype(D))
-    else:
-        assert s.engine_layout(2).size == 0 and s.engine_layout(8).size == 0
-        pq = None
+    pq = s.engine_layout(2).view(spq_dtype(D))
+    refs = s.engine_layout(8 if D == 3 else 11).view(REFCONE if D == 3 else REFCONE2)
+    if D == 2:
+        assert s.engine_layout(8).size == 0
         # SNCH node records of the reference (box, cone axis, half-angle, radius) in node order;
         # children of pair records are matched to them through the subtree primitive ranges below
         snd = s.node_data().reshape(-1, 13)
@@ -152,7 +162,7 @@ def test_snch_pair_layout(name):
     sil_ids = s.engine_layout(7).view(np.int32)
     big = s.engine_layout(9).view(np.uint32)
     ref_sil = s.silhouette_data().reshape(-1, 19 if D == 3 else 10)
-    assert len(pf) == n_pairs and sroot == 0 and (pq is None or len(pq) == n_pairs)
+    assert len(pf) == n_pairs and sroot == 0 and len(pq) == n_pairs
     np.testing.assert_array_equal(sil_ids, ref_sil[:, -1].astype(np.int32))
     if D == 3:
         np.testing.assert_array_equal(sils[:, 0:3], ref_sil[:, 3:6])   # a = m1
@@ -174,7 +184,7 @@ def test_snch_pair_layout(name):
         i = stack.pop()
         for j in range(2):
             ref = int(pf[i]["child"][j])
-            assert pq is None or ref == int(pq[i]["child"][j])
+            assert ref == int(pq[i]["child"][j])
             if ref & LEAF:
                 a, c = leaf_range(ref, big)
                 assert a + c <= len(sils)
@@ -213,10 +223,27 @@ def test_snch_pair_layout(name):
                     al = d @ ax
                     pe = np.abs(d[:, 0] * ax[1] - d[:, 1] * ax[0])
                     assert pf[i]["elo"][j] <= al.min() and al.max() <= pf[i]["ehi"][j] and pe.max() <= pf[i]["eperp"][j]
+                # quantized twin: the reference cone moves to the table, the exact cull is kept
+                check_qpair(pq[i], j, lo, hi, geo.astype(np.float64), None if not (0 < ecos < 1.5) else
+                            (pf[i]["eaxis"][j], ecos, *_normals2d(sils, sl)), ecos > 1.5)
+                has = bool(pq[i]["flags"] & (1 << j))
+                assert has == (pf[i]["ref"][j] == 0)
+                if has:
+                    rc = refs[qref(pq[i], j)]
+                    np.testing.assert_array_equal(rc["lo"], lo)
+                    np.testing.assert_array_equal(rc["hi"], hi)
+                    np.testing.assert_array_equal(rc["axis"], pf[i]["cone_axis"][j])
+                    assert rc["half"] == pf[i]["half"][j] and rc["radius"] == pf[i]["radius"][j]
+                    assert rc["sin"] == pf[i]["sin"][j] and rc["cos"] == pf[i]["cos"][j]
                 continue
-            qlo = decode(pq[i]["qlo"][j], pq[i]["exp"], pq[i]["origin"])
-            qhi = decode(pq[i]["qhi"][j], pq[i]["exp"], pq[i]["origin"])
-            assert (qlo <= lo).all() and (qhi >= hi).all()
+            if D == 3:
+                has = bool(pq[i]["flags"] & (1 << j))
+                assert has == (pf[i]["ref"][j] >= 0)
+                if has:
+                    assert qref(pq[i], j) == pf[i]["ref"][j]
+                    rc = refs[pf[i]["ref"][j]]
+                    np.testing.assert_array_equal(rc["lo"], lo)
+                    np.testing.assert_array_equal(rc["hi"], hi)
             if D == 3 and sl:
                 geo = seg[sl].reshape(-1, 3)
                 assert (geo >= lo).all() and (geo <= hi).all()
@@ -242,7 +269,125 @@ def test_snch_pair_layout(name):
                     a1 /= np.linalg.norm(n1[sl].astype(np.float64), axis=1)
                     sg = np.sign(a0 + a1)
                     assert (sg * a0 >= cos_t - 1e-6).all() and (sg * a1 >= cos_t - 1e-6).all()
-                    # the quantized cone is no narrower
-                    qcos = pq[i]["cs"][j][0] / 32767.0
-                    assert qcos <= cos_t + 1e-6
+                check_qpair(pq[i], j, lo, hi, geo.astype(np.float64),
+                            (ax, cos_t, n0[sl], n1[sl]) if 0 < cos_t < 1.5 else None, cos_t > 1.5)
     assert (covered == 1).all()
+
+
+def qref(rec, j):
+    """Reference-cone table index of child j of a quantized pair (consecutive per pair)."""
+    return int(rec["ref_base"]) + (int(rec["flags"]) & 1 if j == 1 else 0)
+
+
+def _normals2d(sils, sl):
+    A, B, Cc = (sils[sl, k:k + 2].astype(np.float64) for k in (0, 2, 4))
+    n1 = np.stack([-(B - A)[:, 1], (B - A)[:, 0], np.zeros(len(sl))], 1)
+    n2 = np.stack([-(Cc - B)[:, 1], (Cc - B)[:, 0], np.zeros(len(sl))], 1)
+    ok = np.isfinite(n1).all(1) & np.isfinite(n2).all(1)
+    return n1[ok], n2[ok]
+
+
+def check_qpair(rec, j, lo, hi, pts, cone, none):
+    """Child j of a quantized pair record against its float32 child: the decoded box holds the float
+    box; the decoded cone (int16 axis, rounded-up sine) holds every predicate normal pair the float
+    cone holds and is at most 2e-4 rad wider; the extents about the decoded centre, scaled by the
+    decoded half-diagonal, hold every silhouette point.  `cone` = (float axis, cos, n0, n1) when the
+    float cone can cull."""
+    D = len(lo)
+    qlo = decode(rec["qlo"][j], rec["exp"], rec["origin"])
+    qhi = decode(rec["qhi"][j], rec["exp"], rec["origin"])
+    assert (qlo <= lo).all() and (qhi >= hi).all()
+    assert bool(rec["flags"] & (4 << j)) == bool(none)
+    qsin = rec["sin"][j] / 32767.0
+    qcos = 0.0 if rec["sin"][j] >= 32767 else np.sqrt(1.0 - qsin * qsin)
+    a = rec["axis"][j].astype(np.float64) / 32767.0
+    u = a / np.linalg.norm(a) if np.linalg.norm(a) > 0 else a
+    if cone is None:
+        assert none or qcos == 0.0
+    elif qcos > 0:
+        ax, cos_t, n0, n1 = cone
+        assert qcos <= cos_t + 1e-6   # no narrower than the float cone
+        assert np.arccos(qcos) <= np.arccos(min(float(cos_t), 1.0)) + 2e-4
+        u3 = np.zeros(3)
+        u3[:D] = u
+        n0, n1 = np.asarray(n0, np.float64), np.asarray(n1, np.float64)
+        c0 = n0 @ u3 / np.linalg.norm(n0, axis=1)
+        c1 = n1 @ u3 / np.linalg.norm(n1, axis=1)
+        sg = np.sign(c0 + c1)
+        assert (sg * c0 >= qcos - 1e-6).all() and (sg * c1 >= qcos - 1e-6).all()
+    if none or len(pts) == 0:
+        return
+    cen = (np.float32(0.5) * (qlo + qhi)).astype(np.float64)
+    h = (np.float32(0.5) * (qhi - qlo)).astype(np.float64)
+    R = np.sqrt((h * h).sum())
+    d = pts.reshape(-1, D) - cen
+    al = d @ u
+    pe = np.linalg.norm(d - al[:, None] * u, axis=1)
+    tol = 1e-6 * (R + 1e-30)
+    mid, hw = rec["ext_mid"][j] / 32767.0 * R, (rec["ext_half"][j] / 255.0) ** 2 * R
+    assert mid - hw <= al.min() + tol and al.max() <= mid + hw + tol
+    assert pe.max() <= rec["ext_perp"][j] / 255.0 * R + tol
+    # tight: the half-width to one step of its square-root code, the middle to R / 65534
+    need = max(mid - al.min(), al.max() - mid)
+    assert np.sqrt(hw / R) - np.sqrt(max(need, 0.0) / R) <= 1.01 / 255.0 + 1e-6
+    assert abs(mid - 0.5 * (al.min() + al.max())) <= 0.51 * R / 32767.0 + tol
+
+
+@pytest.mark.parametrize("name", ["star300", "star8192", "ico1", "ico4"])
+def test_seed_cuts(name):
+    """The wave-per-query passes' seed cuts: each partitions the tree's primitives / silhouettes
+    below the root (every one under exactly one seed), carries its node's float32 box, and the SNCH
+    cut never passes below a child whose reference cone test can cull."""
+    V, F = SCENES[name]()
+    D = V.shape[1]
+    s = (ext.SNCH if D == 2 else ext.SNCH3D)(V, F)
+    big = s.engine_layout(9).view(np.uint32)
+    f = s.engine_layout(1).view(fnode_dtype(D))
+    pf = s.engine_layout(3).view(spf_dtype(D))
+    for which, n_items in ((12, F.shape[0]), (13, len(s.engine_layout(7).view(np.int32)))):
+        seeds = s.engine_layout(which).view(SEED)
+        assert len(seeds) <= 64
+        if len(seeds) == 0:
+            # no cut: a child of the root pair has a reference cone that can cull
+            assert which == 13 and (pf[0]["ref"] >= 0).any()
+            continue
+        # box of every reachable ref, and whether it sits below a reference cone
+        boxes, below_cone = {}, {}
+        stack = [(0, False)]
+        while stack:
+            i, cone = stack.pop()
+            if which == 12:
+                nd = f[i]
+                kids = [(int(nd["child"][j]), [nd["lo"][k][j] for k in range(D)], [nd["hi"][k][j] for k in range(D)], False)
+                        for j in range(int(nd["nchild"]))]
+            else:
+                kids = [(int(pf[i]["child"][j]), pf[i]["lo"][j], pf[i]["hi"][j], int(pf[i]["ref"][j]) >= 0)
+                        for j in range(2)]
+            for ref, lo, hi, rc in kids:
+                boxes[ref] = (np.float32(lo), np.float32(hi))
+                below_cone[ref] = cone or rc
+                if not ref & LEAF:
+                    stack.append((ref, cone or rc))
+        covered = np.zeros(n_items, np.int64)
+
+        def cover(ref):
+            if ref & LEAF:
+                a, c = leaf_range(ref, big)
+                covered[a:a + c] += 1
+            elif which == 12:
+                for j in range(int(f[ref]["nchild"])):
+                    cover(int(f[ref]["child"][j]))
+            else:
+                cover(int(pf[ref]["child"][0]))
+                cover(int(pf[ref]["child"][1]))
+
+        for e in seeds:
+            ref = int(e["ref"])
+            lo, hi = boxes[ref]
+            np.testing.assert_array_equal(e["lo"][:D], lo)
+            np.testing.assert_array_equal(e["hi"][:D], hi)
+            assert not below_cone[ref]
+            cover(ref)
+        assert (covered == 1).all()
+        if which == 12 and F.shape[0] > 4096:
+            assert len(seeds) > 16   # the cut goes a few levels down on a large tree

@@ -304,20 +304,23 @@ def test_silhouette_traversal_modes(gpu, traversal, name, method):
 
 @pytest.mark.parametrize("traversal", [1, 2])
 @pytest.mark.parametrize("budget", [0, 1])
-@pytest.mark.parametrize("name", ["ico5", "oico4"])
+@pytest.mark.parametrize("name", ["ico5", "oico4", "star4096", "ostar1024"])
 def test_quantized_silhouette_pairs(gpu, traversal, budget, name):
-    """3D silhouettes on the 8-bit pair records (scene option sil_pairs=1; the quantized method
-    otherwise uses the float32 pairs): the same rule against the oracle, and the same distances
-    as the float32 pairs query by query (both layouts only ever keep nodes the reference keeps)."""
+    """Silhouettes on the quantized pair records (scene option sil_pairs=1: 8-bit boxes in 3D,
+    16-bit in 2D, int16 cones, reference cones from a table): the same rule against the oracle,
+    and the same distances as the float32 pairs query by query (both layouts only ever keep nodes
+    the reference keeps)."""
     from gquery_amd.shapes.mesh import Mesh
+    from gquery_amd.shapes.polyline import Polyline
     V, F, base = scene(name)
-    x = synth.uniform_points(3000, 3, seed=43)
-    ref = _oracle("sil", 3, base.snch, x)
-    gap = _gap_from_bruteforce("sil", 3, base.snch, V, F, x, None, ref)
+    dim = V.shape[1]
+    x = synth.uniform_points(3000, dim, seed=43)
+    ref = _oracle("sil", dim, base.snch, x)
+    gap = _gap_from_bruteforce("sil", dim, base.snch, V, F, x, None, ref)
     out = {}
     for pairs in (0, 1):
-        shape = Mesh(V, F, device="cuda:0", traversal=traversal, packet_budget=budget, sil_pairs=pairs,
-                     structure=base.snch)
+        shape = (Mesh if dim == 3 else Polyline)(V, F, device="cuda:0", traversal=traversal, packet_budget=budget,
+                                                 sil_pairs=pairs, structure=base.snch)
         out[pairs] = _np(_query(shape, "sil", "engine", x))
         res = parity.summarize(f"{name}/sil/pairs{pairs}/traversal{traversal}/budget{budget}", out[pairs], ref,
                                gap)
