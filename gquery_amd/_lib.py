@@ -104,6 +104,10 @@ SIGNATURES = {
     "gq_intersect_baseline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_closest_silhouette_baseline": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64, C.POINTER(Records), C.c_void_p]),
     "gq_configure_silhouette": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, _i64p]),
+    "gq_configure_silhouette_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, _i64p,
+                                                 C.c_void_p]),
+    "gq_build_flat_silhouette_edges_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                                        _i64p, C.c_void_p]),
     "gq_structure_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "gq_structure_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "gq_structure_geometry": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.c_void_p, _i64p]),

@@ -31,6 +31,7 @@ HIP_UNITS = {
     "sample.hip": ["-ffp-contract=off"],
     "engine.hip": ["-ffp-contract=off"],
     "gq_api.hip": [],
+    "silprep.hip": ["-ffp-contract=off"],
 }
 CXX_UNITS = {
     # The builder must evaluate float expressions exactly as written (no FMA contraction).

@@ -609,6 +609,42 @@ int gq_configure_silhouette(const float *V, int64_t nv, const int32_t *E, int64_
     return GQ_OK;
 }
 
+static int silprep_status(hipError_t e, uint32_t bad, const char *what) {
+    if (e != hipSuccess) return hip_fail(e, what);
+    if (bad & 1u) return fail(GQ_ERR_INVALID, "index out of range");
+    if (bad & 2u) return fail(GQ_ERR_INVALID, "non-finite vertex coordinate");
+    if (bad & 4u) return fail(GQ_ERR_INVALID, "output buffer too small");
+    return GQ_OK;
+}
+
+static int silprep_args(const float *V, int64_t nv, const int32_t *I, int64_t np, const int64_t *n_floats) {
+    if (!n_floats) return fail(GQ_ERR_INVALID, "n_floats is null");
+    if (nv <= 0 || np <= 0 || !V || !I) return fail(GQ_ERR_INVALID, "empty vertex or index array");
+    if (np >= (1 << 24) || nv >= (1 << 24))
+        return fail(GQ_ERR_INVALID, "more than 2^24 primitives or vertices: indices are stored as float32");
+    return GQ_OK;
+}
+
+int gq_configure_silhouette_device(const float *V, int64_t nv, const int32_t *E, int64_t ne, float *out,
+                                   int64_t *n_floats, void *stream) {
+    int rc = silprep_args(V, nv, E, ne, n_floats);
+    if (rc) return rc;
+    uint32_t bad = 0;
+    const hipError_t e = configure_silhouette_device(V, nv, E, ne, out, out ? *n_floats : 0, n_floats, &bad,
+                                                     (hipStream_t)stream);
+    return silprep_status(e, bad, "configure_silhouette");
+}
+
+int gq_build_flat_silhouette_edges_device(const float *V, int64_t nv, const int32_t *F, int64_t nf, float *out,
+                                          int64_t *n_floats, void *stream) {
+    int rc = silprep_args(V, nv, F, nf, n_floats);
+    if (rc) return rc;
+    uint32_t bad = 0;
+    const hipError_t e = flat_silhouette_edges_device(V, nv, F, nf, out, out ? *n_floats : 0, n_floats, &bad,
+                                                      (hipStream_t)stream);
+    return silprep_status(e, bad, "build_flat_silhouette_edges");
+}
+
 int gq_shape_destroy(gq_shape *shape) {
     if (!shape) return GQ_OK;
     int dev = 0;
@@ -629,15 +665,9 @@ int gq_shape_create(int dim, const float *V, int64_t nv, const int32_t *I, int64
     if (!s) return fail(GQ_ERR_OOM, "host allocation failed");
     hipError_t e = hipGetDevice(&s->device);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-    std::vector<float> sils;
     std::vector<int32_t> vf_start, vf_list;
     try {
-        if (dim == 2) {
-            if (!configure_silhouette_2d(V, nv, I, np, sils, err)) return fail(GQ_ERR_INVALID, err);
-        } else {
-            if (!build_flat_silhouette_edges(V, nv, I, np, sils, err)) return fail(GQ_ERR_INVALID, err);
-            vertex_faces(I, np, nv, vf_start, vf_list);
-        }
+        if (dim == 3) vertex_faces(I, np, nv, vf_start, vf_list);
     } catch (const std::bad_alloc &) {
         return fail(GQ_ERR_OOM, "host allocation failed");
     }
@@ -645,11 +675,29 @@ int gq_shape_create(int dim, const float *V, int64_t nv, const int32_t *I, int64
     v.dim = dim;
     v.n_vertices = nv;
     v.n_prims = np;
-    v.n_sils = (int64_t)sils.size() / (dim == 2 ? 10 : 19);
     int rc;
     if ((rc = upload(s.get(), V, (size_t)(nv * dim), &v.V))) return rc;
     if ((rc = upload(s.get(), I, (size_t)(np * dim), &v.F))) return rc;
-    if ((rc = upload(s.get(), sils.data(), sils.size(), &v.sils))) return rc;
+    {
+        // the silhouette records, built on the device from the uploaded arrays: a counting pass,
+        // then the records into an allocation of exactly that size
+        auto setup = dim == 2 ? configure_silhouette_device : flat_silhouette_edges_device;
+        int64_t nf = 0;
+        uint32_t bad = 0;
+        if ((rc = silprep_status(setup(v.V, nv, v.F, np, nullptr, 0, &nf, &bad, nullptr), bad, "silhouette setup")))
+            return rc;
+        v.n_sils = nf / (dim == 2 ? 10 : 19);
+        void *sp = nullptr;
+        if (nf > 0) {
+            if ((e = hipMalloc(&sp, (size_t)nf * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc");
+            s->allocs.push_back(sp);
+            s->bytes += (size_t)nf * sizeof(float);
+            if ((rc = silprep_status(setup(v.V, nv, v.F, np, (float *)sp, nf, &nf, &bad, nullptr), bad,
+                                     "silhouette setup")))
+                return rc;
+        }
+        v.sils = (const float *)sp;
+    }
     if (dim == 3) {
         if ((rc = upload(s.get(), vf_start.data(), vf_start.size(), &v.vf_start))) return rc;
         if ((rc = upload(s.get(), vf_list.data(), vf_list.size(), &v.vf_list))) return rc;

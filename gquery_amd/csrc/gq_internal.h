@@ -95,6 +95,12 @@ struct ShapeView {
 hipError_t launch_baseline(QueryKind kind, const ShapeView &s, const QueryArgs &q, const float *normal,
                            const uint8_t *on_boundary, hipStream_t st);
 hipError_t launch_normals(const ShapeView &s, float *face_normals, float *vertex_normals, hipStream_t st);
+// Silhouette setups on the device (silprep.hip).  *bad receives 1 (index out of range), 2 (non-finite
+// coordinate) or 4 (out holds fewer than *n_floats floats); they wait for `st` before returning.
+hipError_t configure_silhouette_device(const float *V, int64_t nv, const int32_t *E, int64_t ne, float *out,
+                                       int64_t capacity, int64_t *n_floats, uint32_t *bad, hipStream_t st);
+hipError_t flat_silhouette_edges_device(const float *V, int64_t nv, const int32_t *F, int64_t nf, float *out,
+                                        int64_t capacity, int64_t *n_floats, uint32_t *bad, hipStream_t st);
 
 void set_error(const std::string &msg);
 

@@ -158,8 +158,49 @@ def load(path) -> _Structure:
     return obj
 
 
-def build_flat_silhouette_edges(vertices, faces) -> np.ndarray:
-    """gquery_ext.build_flat_silhouette_edges: all mesh edges as 19-float silhouette records."""
+def _device_setup(fn, vertices, index, dim, stride, what):
+    """Runs a silhouette setup on the device for CUDA tensors (rows, as _as_rows reads them);
+    returns the records as a (k, stride) float32 tensor on that device."""
+    import torch
+    from .fwd import SoA
+    from .scene import as_points
+    dev = (vertices.value if isinstance(vertices, SoA) else vertices).device
+    v = as_points(vertices, dim, dev).contiguous()
+    if isinstance(index, SoA):
+        index = index.rows()
+    i = index if isinstance(index, torch.Tensor) else torch.as_tensor(np.asarray(index))
+    i = i.to(device=dev, dtype=torch.int32)
+    if i.ndim == 2 and i.shape[0] == dim and i.shape[1] != dim:
+        i = i.T
+    i = i.reshape(-1, dim).contiguous()
+    L = _lib.lib()
+    stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    n = C.c_int64(0)
+    _lib.check(getattr(L, fn)(v.data_ptr(), v.shape[0], i.data_ptr(), i.shape[0], None, C.byref(n), stream), what)
+    out = torch.empty(max(n.value, 1), dtype=torch.float32, device=dev)
+    _lib.check(getattr(L, fn)(v.data_ptr(), v.shape[0], i.data_ptr(), i.shape[0], out.data_ptr(), C.byref(n), stream),
+               what)
+    return out[:n.value].reshape(-1, stride)
+
+
+def _on_device(a):
+    try:
+        import torch
+        from .fwd import SoA
+        if isinstance(a, SoA):
+            a = a.value
+        return isinstance(a, torch.Tensor) and a.is_cuda
+    except ImportError:  # pragma: no cover
+        return False
+
+
+def build_flat_silhouette_edges(vertices, faces):
+    """gquery_ext.build_flat_silhouette_edges: all mesh edges as 19-float silhouette records (a flat
+    float32 array).  CUDA tensor vertices run the setup on their device and return a flat device
+    tensor, bit-identical to the host result."""
+    if _on_device(vertices):
+        return _device_setup("gq_build_flat_silhouette_edges_device", vertices, faces, 3, 19,
+                             "build_flat_silhouette_edges").reshape(-1)
     v = _as_rows(vertices, 3, np.float32)
     f = _as_rows(faces, 3, np.int32)
     n = C.c_int64(0)
@@ -171,9 +212,12 @@ def build_flat_silhouette_edges(vertices, faces) -> np.ndarray:
     return out
 
 
-def configure_silhouette(vertices, edges) -> np.ndarray:
-    """Polyline.configure_silhouette (gquery/shapes/polyline.py:44-122) on the host: the polyline's
-    silhouette vertices as (k, 10) float32 records (a, b, c, prev, vertex, next, index)."""
+def configure_silhouette(vertices, edges):
+    """Polyline.configure_silhouette (gquery/shapes/polyline.py:44-122): the polyline's silhouette
+    vertices as (k, 10) float32 records (a, b, c, prev, vertex, next, index).  CUDA tensor vertices
+    run it on their device (a tensor there, bit-identical to the host result)."""
+    if _on_device(vertices):
+        return _device_setup("gq_configure_silhouette_device", vertices, edges, 2, 10, "configure_silhouette")
     v = _as_rows(vertices, 2, np.float32)
     e = _as_rows(edges, 2, np.int32)
     n = C.c_int64(0)

@@ -48,9 +48,12 @@ class Mesh:
 
     @property
     def silhouettes(self):
-        """The mesh's silhouette edges (mesh.py:26-29): build_flat_silhouette_edges records."""
+        """The mesh's silhouette edges (mesh.py:26-29): build_flat_silhouette_edges records, built on
+        the mesh's device (a flat float32 tensor there; 19 floats per edge)."""
         if self._silhouettes is None:
-            self._silhouettes = ext.build_flat_silhouette_edges(self.vertices, self.indices)
+            import torch
+            self._silhouettes = ext.build_flat_silhouette_edges(torch.from_numpy(self.vertices).to(self._device),
+                                                                torch.from_numpy(self.indices).to(self._device))
         return self._silhouettes
 
     def intersect(self, p, v, n=None, on_boundary=False, r_max=INF):

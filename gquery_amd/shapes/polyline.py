@@ -32,8 +32,11 @@ class Polyline:
     def configure_silhouette(self):
         """Polyline.configure_silhouette (polyline.py:44-122): the silhouette vertices (vertices with
         both neighbours, nearly collinear ones dropped) as (k, 10) float32 records a, b, c, prev,
-        vertex, next, index -- the reference's SilhouetteVertices layout."""
-        self.silhouettes = ext.configure_silhouette(self.vertices, self.indices)
+        vertex, next, index -- the reference's SilhouetteVertices layout -- built on the
+        polyline's device (a tensor there)."""
+        import torch
+        self.silhouettes = ext.configure_silhouette(torch.from_numpy(self.vertices).to(self._device),
+                                                    torch.from_numpy(self.indices).to(self._device))
         return self.silhouettes
 
     def _brute_force(self) -> BruteForce:

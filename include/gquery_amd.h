@@ -257,6 +257,19 @@ int gq_closest_silhouette_baseline(gq_shape *shape, const float *x, const float 
 int gq_configure_silhouette(const float *vertices, int64_t n_vertices, const int32_t *edges,
                             int64_t n_edges, float *out, int64_t *n_floats);
 
+/* The two silhouette setups on the device, bit-identical to gq_configure_silhouette and
+ * gq_build_flat_silhouette_edges (Polyline.configure_silhouette, polyline.py:44-122;
+ * build_flat_silhouette_edges, silhouette_edge.cpp:5-71).  vertices, edges / faces and out are
+ * device pointers; out holds *n_floats floats on entry (at most 10 * n_vertices are written in 2D,
+ * 19 * 3 * n_faces in 3D) or is NULL to get the count.  On return *n_floats is the float count
+ * (also when out is too small, which fails with GQ_ERR_INVALID).  Inputs are validated on the
+ * device as the host versions validate them.  Runs on `stream` with stream-ordered scratch and
+ * waits for it before returning. */
+int gq_configure_silhouette_device(const float *vertices, int64_t n_vertices, const int32_t *edges,
+                                   int64_t n_edges, float *out, int64_t *n_floats, void *stream);
+int gq_build_flat_silhouette_edges_device(const float *vertices, int64_t n_vertices, const int32_t *faces,
+                                          int64_t n_faces, float *out, int64_t *n_floats, void *stream);
+
 /* ---- Engine layout inspection (no counterpart in the reference: exposes the host-built engine
  * arrays so tests can check them on machines without a GPU) ---- */
 

@@ -59,7 +59,7 @@ def test_baselines_bit_exact(gpu, name):
     for r_max in (float("inf"), r_arr):
         got = _np(shape.intersect_baseline(x, v, r_max=torch.as_tensor(r_max) if np.ndim(r_max) else r_max))
         _same(got, O.intersect_baseline(dim, V, F, x, v, r_max), FIELDS)
-        sils = shape.silhouettes
+        sils = shape.silhouettes.cpu().numpy()
         got = _np(shape.closest_silhouette_baseline(x, r_max=torch.as_tensor(r_max) if np.ndim(r_max) else r_max))
         _same(got, O.silhouette_baseline(dim, sils, x, r_max), FIELDS)
     # Polyline/Mesh.intersect: origins on the boundary move by -1e-5 n before the brute force
