@@ -25,6 +25,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -2229,13 +2230,85 @@ void launch_group(const Group &G, const EngineParams *J, const uint2 *tiles, uin
     }
 }
 
-// Enqueues the sort passes and every group's traversal.  `tiles_null` lets a single job use the
-// implicit tile table (tile_at with a null table) so the single-call path uploads nothing.
-void launch_groups(const Plan &plan, uint8_t *base, const uint2 *tiles, hipStream_t st, const char *prof_name) {
+// Auxiliary streams of the calling thread on the current device (created once, default priority,
+// kept until the thread ends): with the caller's stream, the four hardware queues a process gets.
+// Per thread, so a stream that joins one thread's graph capture is never used by another thread.
+constexpr int kAuxStreams = 3;
+hipError_t aux_streams(hipStream_t (&out)[kAuxStreams]) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    thread_local std::map<int, std::array<hipStream_t, kAuxStreams>> pool;
+    auto it = pool.find(dev);
+    if (it == pool.end()) {
+        std::array<hipStream_t, kAuxStreams> a{};
+        for (auto &x : a)
+            if ((e = hipStreamCreateWithFlags(&x, hipStreamNonBlocking)) != hipSuccess) return e;
+        it = pool.emplace(dev, a).first;
+    }
+    for (int k = 0; k < kAuxStreams; ++k) out[k] = it->second[k];
+    return hipSuccess;
+}
+
+// Relative cost of a group's tiles, for spreading groups over the streams.
+double group_weight(const Group &G) {
+    const double w = G.kind == kIntersect ? 0.3 : G.kind == kClosestPoint ? 1.0 : (G.dim == 3 ? 4.0 : 1.0);
+    return w * (double)G.n_tiles;
+}
+
+// Enqueues every group's traversal.  Groups are independent (their own tiles, deferred lists and
+// counters), so with more than one they are spread over `st` and the auxiliary streams (largest
+// first, each to the least loaded stream): the auxiliary streams wait for everything enqueued on
+// `st` so far and `st` waits for them at the end, so one group's tail -- its last waves and the
+// draining wave-per-query pass -- overlaps the other groups' work.  Under a graph capture the
+// fork and join record parallel branches.  A null `tiles` lets a single job use the implicit tile
+// table (tile_at with a null table) so the single-call path uploads nothing.
+hipError_t launch_groups(const Plan &plan, uint8_t *base, const uint2 *tiles, hipStream_t st, const char *prof_name) {
     const EngineParams *J = (const EngineParams *)base;
     if (prof_name) prof_begin(st);
-    for (const Group &G : plan.groups) launch_group(G, J, tiles, base, st);
+    const size_t ng = plan.groups.size();
+    hipStream_t aux[kAuxStreams];
+    hipEvent_t fork = nullptr, join[kAuxStreams] = {};
+    bool forked = ng > 1 && aux_streams(aux) == hipSuccess &&
+                  hipEventCreateWithFlags(&fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; forked && k < kAuxStreams; ++k)
+        forked = hipEventCreateWithFlags(&join[k], hipEventDisableTiming) == hipSuccess;
+    if (forked) forked = hipEventRecord(fork, st) == hipSuccess;
+    hipError_t err = hipSuccess;
+    if (!forked) {
+        for (const Group &G : plan.groups) launch_group(G, J, tiles, base, st);
+    } else {
+        std::vector<size_t> order(ng);
+        for (size_t g = 0; g < ng; ++g) order[g] = g;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            return group_weight(plan.groups[a]) > group_weight(plan.groups[b]);
+        });
+        double load[kAuxStreams + 1] = {};
+        bool waited[kAuxStreams] = {};
+        for (size_t g : order) {
+            int sx = 0;
+            for (int k = 1; k <= kAuxStreams; ++k)
+                if (load[k] < load[sx]) sx = k;
+            load[sx] += group_weight(plan.groups[g]) + 1.0;
+            hipStream_t gs = st;
+            if (sx > 0) {
+                if (!waited[sx - 1]) waited[sx - 1] = hipStreamWaitEvent(aux[sx - 1], fork, 0) == hipSuccess;
+                if (waited[sx - 1]) gs = aux[sx - 1];   // a failed wait keeps the group on st
+            }
+            launch_group(plan.groups[g], J, tiles, base, gs);
+        }
+        for (int k = 0; k < kAuxStreams; ++k) {
+            if (!waited[k]) continue;
+            hipError_t e = hipEventRecord(join[k], aux[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, join[k], 0);
+            if (e != hipSuccess && err == hipSuccess) err = e;
+        }
+    }
+    if (fork) hipEventDestroy(fork);
+    for (hipEvent_t e : join)
+        if (e) hipEventDestroy(e);
     if (prof_name) prof_end(st, prof_name);
+    return err;
 }
 
 // Counting sort of every job with at least kSortMin queries, then the traversal of every group.
@@ -2273,8 +2346,9 @@ hipError_t launch_plan(const Plan &plan, uint8_t *base, bool implicit_tiles, hip
         if (max_cells > 0)
             hipLaunchKernelGGL(k_sort_refine, dim3((max_cells + kBlock - 1) / kBlock, nj), dim3(kBlock), 0, st, J);
     }
-    launch_groups(plan, base, tiles, st, prof_name);
-    return hipGetLastError();
+    const hipError_t e = launch_groups(plan, base, tiles, st, prof_name);
+    const hipError_t l = hipGetLastError();
+    return e != hipSuccess ? e : l;
 }
 
 } // namespace
@@ -2328,8 +2402,8 @@ hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, cons
                                           w.dim == 3 ? 30 : 32, st);
     }
     if (e == hipSuccess) {
-        launch_groups(plan, base, nullptr, st, names[w.dim - 2][kind]);
-        e = hipGetLastError();
+        e = launch_groups(plan, base, nullptr, st, names[w.dim - 2][kind]);
+        if (e == hipSuccess) e = hipGetLastError();
     }
     const hipError_t f = hipFreeAsync(buf, st);
     return e != hipSuccess ? e : f;
