@@ -87,7 +87,9 @@ def cpu_baseline(scene, x, budget_s=10.0):
 
 
 def _pmc_table(d):
-    """{kernel name: {counter: mean value per dispatch}} from rocprofv3 csv counter output."""
+    """{kernel name: {counter: mean value per dispatch, counter + "_max": largest dispatch}} from
+    rocprofv3 csv counter output (a kernel dispatched twice per call, such as a wave-per-query pass
+    with its hand-off pass, has its main dispatch in the _max entry)."""
     import csv
     acc = {}
     for f in Path(d).rglob("*counter_collection.csv"):
@@ -99,6 +101,7 @@ def _pmc_table(d):
     out = {}
     for (kn, cn), per in acc.items():
         out.setdefault(kn, {})[cn] = sum(per.values()) / max(len(per), 1)
+        out[kn][cn + "_max"] = max(per.values()) if per else 0.0
     return out
 
 
