@@ -166,12 +166,14 @@ __device__ __forceinline__ void st(GQ_G float *p, const V3 &v) { gqd::st((float 
 __device__ __forceinline__ void st(GQ_G float *p, const V2 &v) { gqd::st((float *)p, v); }
 
 __device__ __forceinline__ RefCone ldg(const GQ_G RefCone *p) {
-    const GQ_G float *f = (const GQ_G float *)p;
+    const GQ_G float4 *f = (const GQ_G float4 *)p;
+    const float4 a = ldg(f), b = ldg(f + 1), c = ldg(f + 2), d = ldg(f + 3);
     RefCone r;
-    r.axis[0] = f[0]; r.axis[1] = f[1]; r.axis[2] = f[2];
-    r.half = f[3]; r.radius = f[4];
-    r.lo[0] = f[5]; r.lo[1] = f[6]; r.lo[2] = f[7];
-    r.hi[0] = f[8]; r.hi[1] = f[9]; r.hi[2] = f[10];
+    r.axis[0] = a.x; r.axis[1] = a.y; r.axis[2] = a.z;
+    r.half = a.w; r.radius = b.x;
+    r.lo[0] = b.y; r.lo[1] = b.z; r.lo[2] = b.w;
+    r.hi[0] = c.x; r.hi[1] = c.y; r.hi[2] = c.z;
+    r.sin_h = c.w; r.cos_h = d.x;
     return r;
 }
 
@@ -1227,6 +1229,40 @@ __device__ __forceinline__ bool ref_cone_2d(const PairChild &c, V2 x, float r, f
     return snch_overlaps(lo, hi, v2(c.ref_axis[0], c.ref_axis[1]), c.half, c.radius, x, r, d_min);
 }
 
+// 3D reference cone test with the same fast path as ref_cone_2d: with c the cosine between the cone
+// axis and the view direction to the box centre and rho = radius / l, the reference's tests are,
+// in exact arithmetic, |c| <= sin(half), then rho > cos(half) or
+// |c| <= sin(half) sqrt(1 - rho^2) + cos(half) rho (rho < 1).  Outside a 4e-6 band around each
+// threshold the outcome equals the reference's float32 evaluation (its rounding, acos and asin
+// included, moves c by under 1e-6: the band is four times that), which runs for everything else
+// and for a query inside the node's bounding sphere.  Every table entry has 0 <= half <= pi/2.
+__device__ __forceinline__ bool ref_cone_3d(const RefCone &rc, const V3 &x, float r, float &d_min) {
+    const V3 lo = v3(rc.lo[0], rc.lo[1], rc.lo[2]), hi = v3(rc.hi[0], rc.hi[1], rc.hi[2]);
+    const V3 axis = v3(rc.axis[0], rc.axis[1], rc.axis[2]);
+    float d_max;
+    box_distance(lo, hi, x, d_min, d_max);
+    if (!(d_min <= r)) return false;
+    constexpr float m = 4e-6f;
+    const V3 va = (lo + hi) / 2.0f - x;
+    const float l2 = dot(va, va);
+    const float il = rsqrtf(l2);
+    const float cs = fabsf(dot(axis, va)) * il;   // NaN at l = 0: the float32 evaluation decides
+    if (cs <= rc.sin_h - m) return true;
+    if (cs >= rc.sin_h + m) {
+        const float l = l2 * il;
+        if (l > rc.radius * (1.0f + m)) {
+            const float rho = rc.radius * il;
+            if (rho >= rc.cos_h + m) return true;
+            if (rho <= rc.cos_h - m) {
+                const float t2 = rc.sin_h * sqrtf(fmaxf(1.0f - rho * rho, 0.0f)) + rc.cos_h * rho;
+                if (cs <= t2 - m) return true;
+                if (cs >= t2 + m) return false;
+            }
+        }
+    }
+    return snch_overlaps(lo, hi, axis, rc.half, rc.radius, x, r, d_min);
+}
+
 template <int D, bool Q>
 __device__ __forceinline__ bool ref_cone_pass(const GP &P, const PairChild &c, const typename VT<D>::T &x, float r,
                                               float &d_min) {
@@ -1246,8 +1282,7 @@ __device__ __forceinline__ bool ref_cone_pass(const GP &P, const PairChild &c, c
             return ref_cone_2d(c, x, r, d_min);
         }
     } else {
-        const RefCone rc = ldg(P.ref_cones + c.ref_cone);
-        return snch_overlaps(ldv<D>(rc.lo), ldv<D>(rc.hi), ldv<D>(rc.axis), rc.half, rc.radius, x, r, d_min);
+        return ref_cone_3d(ldg(P.ref_cones + c.ref_cone), x, r, d_min);
     }
 }
 

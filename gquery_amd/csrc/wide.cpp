@@ -541,6 +541,8 @@ void build_snch_pairs(const HostStructure &h, const std::vector<BinNode> &b, Wid
                 for (int k = 0; k < D; ++k) { rc.axis[k] = n[2 * D + k]; rc.lo[k] = n[k]; rc.hi[k] = n[D + k]; }
                 rc.half = half;
                 rc.radius = n[3 * D + 1];
+                rc.sin_h = (float)std::sin((double)half);
+                rc.cos_h = (float)std::cos((double)half);
                 pf.ref_cone[j] = (int32_t)out.ref_cones.size();
                 pq.flags |= (uint8_t)(kPairRef0 << j);
                 out.ref_cones.push_back(rc);

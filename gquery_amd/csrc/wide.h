@@ -186,11 +186,14 @@ static_assert(offsetof(FNode<2>, child) == 64 && offsetof(FNode<2>, nchild) == 8
 
 // Reference normal cone (snch.cpp:102-154 values, float32) of a node whose cone test can cull
 // (0 <= half_angle <= pi/2); tested exactly as SNCHNode.overlaps does.
-struct RefCone {
+struct alignas(16) RefCone {
     float axis[3];
     float half, radius;
     float lo[3], hi[3];   // the node's box (the reference test uses its centroid and p_max)
+    float sin_h, cos_h;   // sin and cos of the half-angle (the kernels' fast path)
+    float pad[3];
 };
+static_assert(sizeof(RefCone) == 64, "RefCone is four float4");
 // 2D reference cone of a child of a quantized pair record: the float32 box and cone the 2D pair
 // record carries inline (three float4).
 struct RefCone2 {

@@ -34,7 +34,8 @@ def spq_dtype(D):
                      ("ext_perp", "u1", 2), ("pad", "u1", 4)])
 
 
-REFCONE = np.dtype([("axis", "<f4", 3), ("half", "<f4"), ("radius", "<f4"), ("lo", "<f4", 3), ("hi", "<f4", 3)])
+REFCONE = np.dtype([("axis", "<f4", 3), ("half", "<f4"), ("radius", "<f4"), ("lo", "<f4", 3), ("hi", "<f4", 3),
+                    ("sin", "<f4"), ("cos", "<f4"), ("pad", "<f4", 3)])
 REFCONE2 = np.dtype([("lo", "<f4", 2), ("hi", "<f4", 2), ("axis", "<f4", 2), ("half", "<f4"), ("radius", "<f4"),
                      ("sin", "<f4"), ("cos", "<f4"), ("pad", "<f4", 2)])
 SEED = np.dtype([("lo", "<f4", 3), ("ref", "<u4"), ("hi", "<f4", 3), ("pad", "<f4")])
@@ -244,6 +245,9 @@ def test_snch_pair_layout(name):
                     rc = refs[pf[i]["ref"][j]]
                     np.testing.assert_array_equal(rc["lo"], lo)
                     np.testing.assert_array_equal(rc["hi"], hi)
+                    assert 0.0 <= rc["half"] <= np.float32(np.pi / 2)
+                    assert abs(rc["sin"] - np.sin(np.float64(rc["half"]))) <= 1e-7
+                    assert abs(rc["cos"] - np.cos(np.float64(rc["half"]))) <= 1e-7
             if D == 3 and sl:
                 geo = seg[sl].reshape(-1, 3)
                 assert (geo >= lo).all() and (geo <= hi).all()
