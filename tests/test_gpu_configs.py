@@ -110,8 +110,12 @@ def test_config2_icosphere_s6_full(gpu):
     eng = _np(shape.intersect_bvh(xt, vt, method="engine"))
     pl = _np(shape.intersect_bvh(xt, vt, method="plain"))
     ev, pv = eng["valid"].astype(bool), pl["valid"].astype(bool)
-    assert (ev != pv).mean() < 1e-4
     both = ev & pv
+    # the counts the tolerances below allow, printed for the record
+    print(f"config2/ray: {(ev != pv).sum()} of {N_FULL} rays flip hit/miss against the plain kernel; "
+          f"{(parity.rel_err(eng['d'][both], pl['d'][both]) > parity.TOL_REF).sum()} of {both.sum()} hits differ in d "
+          f"by more than {parity.TOL_REF:g} relative")
+    assert (ev != pv).mean() < 1e-4
     assert np.median(parity.rel_err(eng["d"][both], pl["d"][both])) == 0.0
     assert parity.rel_err(eng["d"][both], pl["d"][both]).max() <= 1e-3
     hit = eng["p"][ev]
