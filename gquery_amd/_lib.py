@@ -64,7 +64,8 @@ class Job(C.Structure):
         ("out", Records),
         ("traversal", C.c_int32),
         ("order", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("shared", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
 

@@ -41,7 +41,8 @@ class Batch:
 
     kind is "cp", "ray" or "sil"; method "engine" (quantized nodes) or "engine_fp32";
     packet_budget / traversal override the scene's settings for that job (0 = scene's); order=1
-    traverses the job in the caller's order (no Morton sort)."""
+    traverses the job in the caller's order (no Morton sort); shared=1 says other launches run
+    beside this one, so a small launch group keeps the full-launch packet budgets."""
 
     def __init__(self, jobs, method: str = "engine", device=None, workspace=None, staging=None):
         """workspace / staging: optional reusable buffers (a uint8 device tensor and a pinned uint8
@@ -80,6 +81,7 @@ class Batch:
             jb.out = r
             jb.traversal = int(j.get("traversal", 0))
             jb.order = int(j.get("order", 0))
+            jb.shared = int(j.get("shared", 0))
         self._jobs = arr
         self._n = len(jobs)
         dev, host = C.c_int64(), C.c_int64()

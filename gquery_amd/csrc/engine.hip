@@ -68,8 +68,13 @@ constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil3
 // queries 1.80 vs 0.48, 16,384 2.29 vs 0.76, 65,536 3.09 vs 1.45, 262,144 3.95 vs 3.90, 1,048,576
 // 7.04 vs 13.4; 65,536-segment star (2D closest point): 16,384 1.06 vs 0.45, 65,536 1.20 vs 1.03,
 // 262,144 1.47 vs 3.35; 3D silhouettes at budget 1: 300 0.42 vs 0.25, 65,536 1.53 vs 1.36.
+// When other launches share the device (gq_job.shared; a serving loop's stream slots) the GPU is
+// not idle, and total work decides throughput and queueing: there the full-launch budgets win
+// (served workload, 2 + 1 stream slots: 140 vs 131 Mq/s saturated, p99 at 100 Mq/s offered 33.4 vs
+// 43.5 ms).
 constexpr int kSmallBudgetCp = 64, kSmallBudgetSil3 = 1;
 constexpr int64_t kSmallLaunchCp3 = 1 << 19, kSmallLaunchCp2 = 1 << 17, kSmallLaunchSil3 = 1 << 18;
+constexpr int64_t kFullLaunch = kSmallLaunchCp3;
 
 // Default packet budget of a launch group of `group_q` queries (no job or scene override).
 int default_budget(int kind, int dim, int64_t group_q) {
@@ -2135,7 +2140,13 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
     for (Group &G : plan.groups) {
         size_t nq = 0;
         int64_t group_q = 0;
-        for (int j : G.jobs) group_q += jobs[j].q.n;
+        bool shared = false;
+        for (int j : G.jobs) {
+            group_q += jobs[j].q.n;
+            shared = shared || jobs[j].shared;
+        }
+        // beside other launches a small group does not leave the GPU idle: budget it as a full one
+        if (shared) group_q = std::max<int64_t>(group_q, kFullLaunch);
         for (int j : G.jobs) {
             EngineParams &P = plan.P[j];
             const WideView &w = *jobs[j].w;

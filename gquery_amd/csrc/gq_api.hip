@@ -466,12 +466,14 @@ static int batch_jobs(int n_jobs, const gq_job *jobs, std::vector<EngineJob> &ou
         if (!engine_supported(kind, b.scene->wide)) return fail(GQ_ERR_UNSUPPORTED, at + "tree deeper than the engine's stack");
         if (b.traversal < 0 || b.traversal > 2) return fail(GQ_ERR_INVALID, at + "traversal must be 0, 1 or 2");
         if (b.order < 0 || b.order > 1) return fail(GQ_ERR_INVALID, at + "order must be 0 or 1");
+        if (b.shared < 0 || b.shared > 1) return fail(GQ_ERR_INVALID, at + "shared must be 0 or 1");
         if (device >= 0 && b.scene->device != device) return fail(GQ_ERR_INVALID, at + "scenes on different devices");
         device = b.scene->device;
         if (b.n == 0) continue;
         QueryArgs q{b.x, b.v, b.r_max, b.r_max_scalar, b.n, b.out};
         out.push_back(EngineJob{&b.scene->wide, kind, quantized_layout(b.method, kind, b.scene->wide), q, b.packet_budget,
-                                b.traversal != 0 ? b.traversal : b.scene->wide.traversal, b.order == 1});
+                                b.traversal != 0 ? b.traversal : b.scene->wide.traversal, b.order == 1,
+                                b.shared == 1});
     }
     return GQ_OK;
 }

@@ -248,6 +248,7 @@ struct EngineJob {
     int budget;
     int traversal;
     bool input_order = false;   // traverse in the caller's order (no Morton sort)
+    bool shared = false;        // other launches share the device: full-launch default budgets
 };
 
 // One batch on `st`: scratch comes from the stream-ordered allocator and is released after use.

@@ -207,6 +207,10 @@ class GpuExecutor:
         jobs = [(self.shapes[r.scene], r.kind, r.x, r.v) for r in reqs]
         dev, host = Batch.sizes(jobs, method=self.method)
         buf = self._take_buffers(dev, host)
+        # launches overlap on the device whenever there is more than one slot: mark the jobs shared
+        # so small launch groups keep the full-launch packet budgets (less total work)
+        shared = int(sum(len(p) for p in self.streams) > 1)
+        jobs = [dict(shape=s, kind=k, x=x, v=v, shared=shared) for s, k, x, v in jobs]
         # the request inputs were written on the default stream: order this launch after them
         st.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(st):

@@ -207,7 +207,13 @@ typedef struct gq_job {
     int32_t order;           /* 0 = Morton-sort closest-point and silhouette batches of 8,192 or
                                 more queries (the default); 1 = traverse in the caller's order
                                 (queries already coherent, or to measure the unsorted case)     */
-    int32_t reserved[2];
+    int32_t shared;          /* 0 = this enqueue has the device to itself: a launch group too
+                                small to fill the GPU moves its work to the wave-per-query pass
+                                early, which shortens its latency; 1 = other launches run beside
+                                it (a serving loop's stream slots): default packet budgets are
+                                those of a full launch, the cheaper total work.  Results do not
+                                change                                                           */
+    int32_t reserved[1];
 } gq_job;
 
 /* Bytes of device workspace and of pinned host staging that gq_batch_enqueue needs for `jobs`. */

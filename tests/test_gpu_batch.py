@@ -51,6 +51,18 @@ def test_batch_equals_single_calls(gpu, method):
             assert np.array_equal(g[f].view(np.uint8), ref[f].view(np.uint8)), (kind, f)
 
 
+def test_shared_jobs_equal_single_calls(gpu):
+    """shared=1 (other launches beside this one) changes only the packet budgets of small launch
+    groups, never a record."""
+    jobs = _jobs(2)
+    got = Batch([dict(shape=s, kind=k, x=x, v=v, shared=1) for s, k, x, v in jobs]).run()
+    for (shape, kind, x, v), r in zip(jobs, got):
+        ref = _np(_single(shape, kind, x, v, "engine"))
+        g = _np(r)
+        for f in ref:
+            assert np.array_equal(g[f].view(np.uint8), ref[f].view(np.uint8)), (kind, f)
+
+
 def test_batch_graph_replays(gpu):
     jobs = _jobs(1)
     b = Batch(jobs)
@@ -77,6 +89,8 @@ def test_batch_rejects_bad_jobs(gpu):
         Batch([(shape, "cp", x)], method="plain")
     with pytest.raises(ValueError):
         Batch([{"shape": shape, "kind": "cp", "x": x, "traversal": 7}])
+    with pytest.raises(ValueError):
+        Batch([{"shape": shape, "kind": "cp", "x": x, "shared": 2}])
 
 
 def test_batch_input_order_equals_sorted(gpu):
