@@ -99,6 +99,7 @@ constexpr float kAbsSlack = 32.0f * 5.9604644775390625e-08f;
 
 struct EngineParams {
     const void *nodes;        // BVH4 (FNode / QNode) or SNCH pairs (SPairF / SPairQ)
+    const void *cyl;          // closest point: CBound per BVH4 node (the node format's own)
     const float4 *prims;
     const int32_t *ids;
     const RefCone *ref_cones;
@@ -136,6 +137,7 @@ struct GRecords {
 };
 struct GP {
     const GQ_G uint8_t *nodes;
+    const GQ_G uint8_t *cyl;
     const GQ_G float4 *prims;
     const GQ_G int32_t *ids;
     const GQ_G RefCone *ref_cones;
@@ -186,6 +188,7 @@ __device__ __forceinline__ RefCone ldg(const GQ_G RefCone *p) {
 __device__ __forceinline__ GP gview(const EngineParams &in) {
     GP P;
     P.nodes = gcast((const uint8_t *)in.nodes);
+    P.cyl = gcast((const uint8_t *)in.cyl);
     P.prims = gcast(in.prims);
     P.ids = gcast(in.ids);
     P.ref_cones = gcast(in.ref_cones);
@@ -222,30 +225,27 @@ template <int D> __device__ __forceinline__ float coord_slack(const GP &P, const
     for (int k = 0; k < D; ++k) s += fabsf(xc[k]) + fmaxf(fabsf(P.root_lo[k]), fabsf(P.root_hi[k]));
     return kAbsSlack * s;
 }
-// Squared pruning bound for a closest-point answer at distance d: boxes farther than this cannot
-// hold a primitive whose float32 distance is below or equal to d.
+// Squared pruning bound for a closest-point answer at distance d: a child or triangle whose
+// squared lower bound exceeds this cannot hold a point whose float32 distance is below or equal to d.
 __device__ __forceinline__ float cp_bound2(float d, float slack) {
     const float b = fmaf(d, 1.000001f, slack);
     return b * b;
 }
 
-// Lower bounds on the distance from p to triangle abc that decide whether the exact leaf test can
-// matter: the triangle's own bounding box (squared distance against bound2, the test a one-triangle
-// box would get) and its supporting plane (pl: unit normal, offset; against best, widened for
-// rounding).  Either proves the float32 leaf distance exceeds best.  The plane alone is weak -- a
-// far triangle of a flat patch has its plane near every query -- while the box is tight; together
-// they skip most exact tests of a leaf that only some lanes need.
+// Lower bound on the distance from p to triangle abc that decides whether the exact leaf test can
+// matter: its circumscribing disk (centre m in the free lanes of a, b, c; unit normal n and radius
+// in pl), sqrt(h^2 + max(|q| - rad, 0)^2) with h = (p - m).n and q = (p - m) - h n, squared, against
+// bound2.  Near the true distance wherever the query is not right above the triangle: a query at
+// distance d from a smooth surface sees many triangles whose box or supporting plane is within d,
+// and the disk rejects nearly all of them (on a 327,680-triangle sphere-like mesh, 4.7 triangles per
+// uniform query pass at the final distance, against 239 for the box and plane together).
 __device__ __forceinline__ bool tri_may_beat(const float *pc, const float4 &a, const float4 &b, const float4 &c,
-                                             const float4 &pl, float bound2, float best, float slack) {
-    const float lo0 = fminf(fminf(a.x, b.x), c.x), hi0 = fmaxf(fmaxf(a.x, b.x), c.x);
-    const float lo1 = fminf(fminf(a.y, b.y), c.y), hi1 = fmaxf(fmaxf(a.y, b.y), c.y);
-    const float lo2 = fminf(fminf(a.z, b.z), c.z), hi2 = fmaxf(fmaxf(a.z, b.z), c.z);
-    const float o0 = fmaxf(fmaxf(lo0 - pc[0], pc[0] - hi0), 0.0f);
-    const float o1 = fmaxf(fmaxf(lo1 - pc[1], pc[1] - hi1), 0.0f);
-    const float o2 = fmaxf(fmaxf(lo2 - pc[2], pc[2] - hi2), 0.0f);
-    const float d2 = fmaf(o2, o2, fmaf(o1, o1, o0 * o0));
-    const float lb = fabsf(fmaf(pl.x, pc[0], fmaf(pl.y, pc[1], fmaf(pl.z, pc[2], -pl.w))));
-    return (d2 <= bound2) & (fmaf(lb, 0.99999f, -slack) <= best);
+                                             const float4 &pl, float bound2) {
+    const float d0 = pc[0] - a.w, d1 = pc[1] - b.w, d2 = pc[2] - c.w;
+    const float h = fmaf(d2, pl.z, fmaf(d1, pl.y, d0 * pl.x));
+    const float q0 = fmaf(-h, pl.x, d0), q1 = fmaf(-h, pl.y, d1), q2 = fmaf(-h, pl.z, d2);
+    const float m = fmaxf(__builtin_amdgcn_sqrtf(fmaf(q2, q2, fmaf(q1, q1, q0 * q0))) - pl.w, 0.0f);
+    return fmaf(h, h, m * m) <= bound2;
 }
 
 // ---- lane-per-query stacks.  In LDS: depth chosen per launch from the tree depth, one column per
@@ -396,6 +396,51 @@ __device__ __forceinline__ void load_node4(const void *base, uint32_t idx, Node4
     }
 }
 
+// Bounding cylinders of a BVH4 node's children (CBound<D>, the first 48 bytes of its 64): every
+// field is the upper half of a float32 pattern, widened with a shift (scalar instructions when the
+// record is wave-uniform).
+template <int D> struct Cyl4 {
+    float ax[D][4], lo[4], hi[4], rad[4];
+};
+template <int D>
+__device__ __forceinline__ void load_cyl4(const void *base, uint32_t idx, Cyl4<D> &cy) {
+    Words<12> w;
+    w.load(reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64);
+    auto half = [&](int i) {
+        const uint32_t word = w.w[i >> 1];
+        return __uint_as_float((i & 1) ? (word & 0xFFFF0000u) : (word << 16));
+    };
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) cy.ax[k][j] = half(4 * k + j);
+        cy.lo[j] = half(4 * D + j);
+        cy.hi[j] = half(4 * D + 4 + j);
+        cy.rad[j] = half(4 * D + 8 + j);
+    }
+}
+// Squared lower bound on the distance from x to anything below child j: x against the child's
+// bounding cylinder about its box centre c (the same float32 centre the host used), s = (x - c).u
+// the position along the axis (returned for the ordering key).
+template <int D, bool Q>
+__device__ __forceinline__ float cyl_lb2(const Node4<D, Q> &nd, const Cyl4<D> &cy, int j, const float *x, float &s) {
+    float dv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) dv[k] = x[k] - 0.5f * (nd.lo[k][j] + nd.hi[k][j]);
+    s = dv[0] * cy.ax[0][j];
+#pragma unroll
+    for (int k = 1; k < D; ++k) s = fmaf(dv[k], cy.ax[k][j], s);
+    float q2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const float q = fmaf(-s, cy.ax[k][j], dv[k]);
+        q2 = fmaf(q, q, q2);
+    }
+    const float sd = fmaxf(fmaxf(cy.lo[j] - s, s - cy.hi[j]), 0.0f);
+    const float m = fmaxf(__builtin_amdgcn_sqrtf(q2) - cy.rad[j], 0.0f);
+    return fmaf(sd, sd, m * m);
+}
+
 __device__ __forceinline__ void cswap(float *k, uint32_t *v, int a, int b) {
     const bool s = k[b] < k[a];
     const float ka = k[a], kb = k[b];
@@ -534,7 +579,7 @@ __device__ __forceinline__ void cp_leaf_compact(const GP &P, PacketLds &L, uint3
             const uint32_t s = c0 + j;
             const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
             const float4 pl = ldg(P.prims + (4 * s + 3));
-            const bool need = live && tri_may_beat(pc, a, b, c, pl, bound2, best, slack);
+            const bool need = live && tri_may_beat(pc, a, b, c, pl, bound2);
             const unsigned long long m = __ballot(need);
             if (need) L.pair[total + lanes_below(m)] = (uint16_t)((j << 6) | (uint32_t)lane);
             total += __popcll(m);
@@ -610,9 +655,12 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
         if (!ref_is_leaf(cur)) {
             Node4<D, Q> nd;
             load_node4<D, Q>((const void *)P.nodes, cur, nd);
+            Cyl4<D> cy;
+            load_cyl4<D>((const void *)P.cyl, cur, cy);
             float key[4];
             uint32_t ref[4];
-            // children any lane must visit; the ordering key (distance from the packet centroid) is
+            // children any lane must visit: within the lane's bound of both the child box and the
+            // child's bounding cylinder; the ordering key (distance from the packet centroid) is
             // computed only when more than one is open (3D: 7.41 vs 7.45 ms; float32 nodes 7.18 vs
             // 7.32 ms)
             bool any[4];
@@ -626,7 +674,9 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
                     const float o = fmaxf(fmaxf(nd.lo[k][j] - pc[k], pc[k] - nd.hi[k][j]), 0.0f);
                     d2 = fmaf(o, o, d2);
                 }
-                any[j] = __ballot((j < nd.n) && (d2 <= bound2)) != 0ull;
+                float sx;
+                const float lb2 = cyl_lb2<D, Q>(nd, cy, j, pc, sx);
+                any[j] = __ballot((j < nd.n) && (d2 <= bound2) && (lb2 <= bound2)) != 0ull;
                 nh += (int)any[j];
                 ref[j] = nd.child[j];
                 if (any[j]) only = ref[j];
@@ -640,13 +690,19 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
                 for (int j = 0; j < 4; ++j) {
                     key[j] = kInf;
                     if (any[j]) {
+                        // box distance of the centroid, or its distance behind the cylinder's slab
+                        // (the side opposite the normals) where larger: the slab's near side would
+                        // order the tilted patches in front of a convex surface first
                         float c2 = 0.0f;
 #pragma unroll
                         for (int k = 0; k < D; ++k) {
                             const float oc = fmaxf(fmaxf(nd.lo[k][j] - cen[k], cen[k] - nd.hi[k][j]), 0.0f);
                             c2 = fmaf(oc, oc, c2);
                         }
-                        key[j] = uniformf(c2);
+                        float sc;
+                        (void)cyl_lb2<D, Q>(nd, cy, j, cen, sc);
+                        const float behind = fmaxf(cy.lo[j] - sc, 0.0f);
+                        key[j] = uniformf(fmaxf(c2, behind * behind));
                     }
                 }
                 sort4(key, ref);
@@ -817,6 +873,8 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                 if (!ref_is_leaf(e.x)) {
                     Node4<D, Q> nd;
                     load_node4<D, Q>((const void *)P.nodes, e.x, nd);
+                    Cyl4<D> cy;
+                    load_cyl4<D>((const void *)P.cyl, e.x, cy);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         float d2 = 0.0f;
@@ -825,8 +883,11 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                             const float o = fmaxf(fmaxf(nd.lo[c][j] - pc[c], pc[c] - nd.hi[c][j]), 0.0f);
                             d2 = fmaf(o, o, d2);
                         }
-                        hit[j] = j < nd.n && d2 <= bound2;
-                        ent[j] = make_uint2(nd.child[j], __float_as_uint(d2));
+                        float sx;
+                        const float lb2 = fmaxf(d2, cyl_lb2<D, Q>(nd, cy, j, pc, sx));
+                        hit[j] = j < nd.n && lb2 <= bound2;
+                        // the entry's key is re-tested against the bound when it is taken
+                        ent[j] = make_uint2(nd.child[j], __float_as_uint(lb2));
                     }
                 } else {
                     uint32_t first, count;
@@ -836,7 +897,7 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                         if constexpr (D == 3) {
                             const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
                             const float4 pl = ldg(P.prims + (4 * s + 3));
-                            if (!tri_may_beat(pc, a, b, c, pl, bound2, best, slack)) continue;
+                            if (!tri_may_beat(pc, a, b, c, pl, bound2)) continue;
                             V3 pt;
                             V2 uv;
                             dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
@@ -2119,6 +2180,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             P.n_seed = w.n_sil_seed;
         } else {
             P.nodes = jb.quantized ? w.qnodes : w.nodes;
+            P.cyl = jb.quantized ? w.cbq : w.cbf;
             P.prims = w.prims;
             P.ids = w.prim_ids;
             P.root = w.root_ref;

@@ -320,6 +320,8 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     }
     if ((rc = upload(s.get(), wh.nodes.data(), wh.nodes.size(), (const uint8_t **)&wv.nodes))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.qnodes.data(), wh.qnodes.size(), (const uint8_t **)&wv.qnodes))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.cbq.data(), wh.cbq.size(), (const uint8_t **)&wv.cbq))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.cbf.data(), wh.cbf.size(), (const uint8_t **)&wv.cbf))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.prims.data(), wh.prims.size(), &wv.prims))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.prim_ids.data(), wh.prim_ids.size(), &wv.prim_ids))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.big_leaves.data(), wh.big_leaves.size(), &wv.big_leaves))) return cleanup(rc);
@@ -532,6 +534,8 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     case 11: src = wh.ref_cones2.data(); bytes = wh.ref_cones2.size() * sizeof(RefCone2); break;
     case 12: src = wh.bvh_seed.data(); bytes = wh.bvh_seed.size() * sizeof(Seed); break;
     case 13: src = wh.sil_seed.data(); bytes = wh.sil_seed.size() * sizeof(Seed); break;
+    case 14: src = wh.cbq.data(); bytes = wh.cbq.size(); break;
+    case 15: src = wh.cbf.data(); bytes = wh.cbf.size(); break;
     default: return fail(GQ_ERR_INVALID, "unknown layout array");
     }
     if (out) {

@@ -34,6 +34,7 @@ struct WideView {
     int64_t n_nodes, n_snch_nodes, n_prims, n_sils;
     const void *nodes;        // FNode<D>[n_nodes]  float32 BVH4
     const void *qnodes;       // QNode<D>[n_nodes]  quantized BVH4
+    const void *cbq, *cbf;    // CBound<D>[n_nodes]  child bounding cylinders about the QNode / FNode box centres
     const void *snodes;       // SPairF<D>[n_snch_nodes]
     const void *qsnodes;      // SPairQ<D>[n_snch_nodes]
     const float4 *prims;      // 3D: 3 float4 per triangle (a, b, c); 2D: 1 float4 per segment (a|b)

@@ -87,8 +87,85 @@ struct LeafTable {
     }
 };
 
+// Upper halves of float32 patterns (the kernels widen them with a 16-bit shift), rounded so the
+// widened value bounds x: dir < 0 below, dir > 0 above, dir == 0 toward zero.
+inline double half_value(uint16_t h) { return (double)u2f((uint32_t)h << 16); }
+inline uint16_t half_step(uint16_t h, bool up) {
+    const bool neg = (h & 0x8000u) != 0;
+    const uint16_t mag = h & 0x7FFFu;
+    if (up) return neg ? (mag == 0 ? (uint16_t)0x0001u : (uint16_t)(h - 1)) : (uint16_t)(h + 1);
+    return neg ? (uint16_t)(h + 1) : (mag == 0 ? (uint16_t)0x8001u : (uint16_t)(h - 1));
+}
+inline uint16_t half_bound(double x, int dir) {
+    const float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    uint16_t h = (uint16_t)(u >> 16);
+    if (dir < 0) {
+        while (half_value(h) > x) h = half_step(h, false);
+    } else if (dir > 0) {
+        while (half_value(h) < x) h = half_step(h, true);
+    } else {
+        while (std::fabs(half_value(h)) > std::fabs(x)) h = (uint16_t)((h & 0x8000u) | ((h & 0x7FFFu) - 1u));
+    }
+    return h;
+}
+
+// Area-weighted normal sum of the reference primitives [pb, pb + pc) (3D cross(b - a, c - a), 2D
+// (dy, -dx)).
+template <int D> void normal_sum(const HostStructure &h, uint32_t pb, uint32_t pc, double *n) {
+    const int ps = kPrimStride[D];
+    n[0] = n[1] = n[2] = 0.0;
+    for (uint32_t s = pb; s < pb + pc; ++s) {
+        const float *r = h.primitive_data.data() + (size_t)s * ps;
+        if constexpr (D == 3) {
+            double e1[3], e2[3];
+            for (int k = 0; k < 3; ++k) { e1[k] = (double)r[3 + k] - r[k]; e2[k] = (double)r[6 + k] - r[k]; }
+            n[0] += e1[1] * e2[2] - e1[2] * e2[1];
+            n[1] += e1[2] * e2[0] - e1[0] * e2[2];
+            n[2] += e1[0] * e2[1] - e1[1] * e2[0];
+        } else {
+            n[0] += (double)r[3] - r[1];
+            n[1] -= (double)r[2] - r[0];
+        }
+    }
+}
+
+// Bounding cylinder of child j (primitives [pb, pb + pc)) about the centre c the kernels compute
+// from its box, in both record arrays' sense: the axis halves, then lo / hi / rad over every vertex
+// in double with the widened axis, rounded outward.
 template <int D>
-void build_bvh4(const std::vector<BinNode> &b, int max_leaf, WideHost &out, LeafTable &lt) {
+void child_cylinder(const HostStructure &h, uint32_t pb, uint32_t pc, const uint16_t *ax, const float *c,
+                    CBound<D> &cb, int j) {
+    const int ps = kPrimStride[D];
+    double u[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < D; ++k) {
+        cb.ax[k][j] = ax[k];
+        u[k] = half_value(ax[k]);
+    }
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo, rad2 = 0.0;
+    for (uint32_t s = pb; s < pb + pc; ++s) {
+        const float *r = h.primitive_data.data() + (size_t)s * ps;
+        for (int v = 0; v < D; ++v) {
+            double dv[3] = {0.0, 0.0, 0.0}, t = 0.0;
+            for (int k = 0; k < D; ++k) {
+                dv[k] = (double)r[v * D + k] - (double)c[k];
+                t += dv[k] * u[k];
+            }
+            double q2 = 0.0;
+            for (int k = 0; k < D; ++k) q2 += (dv[k] - t * u[k]) * (dv[k] - t * u[k]);
+            lo = std::min(lo, t);
+            hi = std::max(hi, t);
+            rad2 = std::max(rad2, q2);
+        }
+    }
+    cb.lo[j] = half_bound(lo, -1);
+    cb.hi[j] = half_bound(hi, 1);
+    cb.rad[j] = half_bound(std::sqrt(rad2), 1);
+}
+
+template <int D>
+void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_leaf, WideHost &out, LeafTable &lt) {
     struct Tmp {
         int n = 0;
         int bin[4];
@@ -138,6 +215,8 @@ void build_bvh4(const std::vector<BinNode> &b, int max_leaf, WideHost &out, Leaf
     out.n_nodes = n;
     out.nodes.assign(n * sizeof(FNode<D>), 0);
     out.qnodes.assign(n * sizeof(QNode<D>), 0);
+    out.cbq.assign(n * sizeof(CBound<D>), 0);
+    out.cbf.assign(n * sizeof(CBound<D>), 0);
     constexpr int L = QBits<D>::levels;
     for (size_t i = 0; i < n; ++i) {
         const Tmp &t = tmp[i];
@@ -145,6 +224,9 @@ void build_bvh4(const std::vector<BinNode> &b, int max_leaf, WideHost &out, Leaf
         std::memset(&fn, 0, sizeof fn);
         QNode<D> qn;
         std::memset(&qn, 0, sizeof qn);
+        CBound<D> cq, cf;
+        std::memset(&cq, 0, sizeof cq);
+        std::memset(&cf, 0, sizeof cf);
         fn.nchild = (uint32_t)t.n;
         qn.nchild = (uint8_t)t.n;
         // node frame = union of the children
@@ -165,6 +247,7 @@ void build_bvh4(const std::vector<BinNode> &b, int max_leaf, WideHost &out, Leaf
             if (j < t.n) {
                 const BinNode &c = b[t.bin[j]];
                 fn.child[j] = qn.child[j] = t.ref[j];
+                float cfc[3] = {0, 0, 0}, cqc[3] = {0, 0, 0};
                 for (int k = 0; k < D; ++k) {
                     fn.lo[k][j] = c.lo[k];
                     fn.hi[k][j] = c.hi[k];
@@ -172,7 +255,20 @@ void build_bvh4(const std::vector<BinNode> &b, int max_leaf, WideHost &out, Leaf
                     quantize(c.lo[k], c.hi[k], nlo[k], e[k], L, ql, qh);
                     qn.qlo[k][j] = (typename QBits<D>::T)ql;
                     qn.qhi[k][j] = (typename QBits<D>::T)qh;
+                    // box centres exactly as the kernels compute them (no contraction here)
+                    const float dlo = decode(ql, e[k], nlo[k]), dhi = decode(qh, e[k], nlo[k]);
+                    const float sq = dlo + dhi, sf = c.lo[k] + c.hi[k];
+                    cqc[k] = 0.5f * sq;
+                    cfc[k] = 0.5f * sf;
                 }
+                double nsum[3];
+                normal_sum<D>(h, c.pbegin, c.pcount, nsum);
+                const double len = std::sqrt(nsum[0] * nsum[0] + nsum[1] * nsum[1] + nsum[2] * nsum[2]);
+                uint16_t ax[3] = {0, 0, 0};
+                if (len > 0.0 && std::isfinite(len))
+                    for (int k = 0; k < D; ++k) ax[k] = half_bound(nsum[k] / len, 0);
+                child_cylinder<D>(h, c.pbegin, c.pcount, ax, cqc, cq, j);
+                child_cylinder<D>(h, c.pbegin, c.pcount, ax, cfc, cf, j);
             } else {
                 // empty slot: an inverted box that no query overlaps
                 for (int k = 0; k < D; ++k) {
@@ -186,6 +282,8 @@ void build_bvh4(const std::vector<BinNode> &b, int max_leaf, WideHost &out, Leaf
         }
         std::memcpy(out.nodes.data() + i * sizeof(FNode<D>), &fn, sizeof fn);
         std::memcpy(out.qnodes.data() + i * sizeof(QNode<D>), &qn, sizeof qn);
+        std::memcpy(out.cbq.data() + i * sizeof(CBound<D>), &cq, sizeof cq);
+        std::memcpy(out.cbf.data() + i * sizeof(CBound<D>), &cf, sizeof cf);
     }
 }
 
@@ -661,7 +759,7 @@ template <int D> void build_seeds(WideHost &out) {
 template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_leaf) {
     std::vector<BinNode> b = read_bvh<D>(h);
     LeafTable lt;
-    build_bvh4<D>(b, max_leaf, out, lt);
+    build_bvh4<D>(h, b, max_leaf, out, lt);
     out.root_ref = 0;
     for (int k = 0; k < D; ++k) { out.scene_lo[k] = b[0].lo[k]; out.scene_hi[k] = b[0].hi[k]; }
     // primitives in leaf order
@@ -671,21 +769,42 @@ template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_
     for (int64_t s = 0; s < h.n_prims; ++s) {
         const float *r = h.primitive_data.data() + s * ps;
         if (D == 3) {
-            out.prims[4 * s + 0] = make_float4(r[0], r[1], r[2], 0.0f);
-            out.prims[4 * s + 1] = make_float4(r[3], r[4], r[5], 0.0f);
-            out.prims[4 * s + 2] = make_float4(r[6], r[7], r[8], 0.0f);
-            // supporting plane (unit normal, offset) for the closest-point prefilter: |n.x - o| is a
-            // lower bound on the distance to the triangle; a degenerate triangle gets n = 0 (no bound)
+            // circumscribing disk for the closest-point prefilter: centre m (the centroid in float32,
+            // in the free lanes of a, b, c), unit normal n (components rounded toward zero, so
+            // |n| <= 1) and the vertices' largest distance from the line m + t n, rounded up.  With
+            // h = (x - m).n and q = (x - m) - h n, sqrt(h^2 + max(|q| - rad, 0)^2) is a lower bound
+            // on the distance from x to the triangle, up to the triangle's float32 thickness along
+            // n, a few ulps of the coordinates; a degenerate triangle gets n = 0 and a sphere.
+            float m[3];
+            for (int k = 0; k < 3; ++k) m[k] = (float)(((double)r[k] + r[3 + k] + r[6 + k]) / 3.0);
             double e1[3], e2[3], n[3];
             for (int k = 0; k < 3; ++k) { e1[k] = (double)r[3 + k] - r[k]; e2[k] = (double)r[6 + k] - r[k]; }
             n[0] = e1[1] * e2[2] - e1[2] * e2[1];
             n[1] = e1[2] * e2[0] - e1[0] * e2[2];
             n[2] = e1[0] * e2[1] - e1[1] * e2[0];
-            double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-            if (l > 0 && std::isfinite(l)) { for (int k = 0; k < 3; ++k) n[k] /= l; }
-            else { n[0] = n[1] = n[2] = 0.0; }
-            double off = n[0] * r[0] + n[1] * r[1] + n[2] * r[2];
-            out.prims[4 * s + 3] = make_float4((float)n[0], (float)n[1], (float)n[2], (float)off);
+            const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            float nf[3] = {0.0f, 0.0f, 0.0f};
+            if (l > 0 && std::isfinite(l)) {
+                for (int k = 0; k < 3; ++k) {
+                    const double v = n[k] / l;
+                    nf[k] = (float)v;
+                    if (std::fabs((double)nf[k]) > std::fabs(v)) nf[k] = std::nextafter(nf[k], 0.0f);
+                }
+            }
+            double rad2 = 0.0;
+            for (int v = 0; v < 3; ++v) {
+                double dv[3], t = 0.0;
+                for (int k = 0; k < 3; ++k) { dv[k] = (double)r[3 * v + k] - m[k]; t += dv[k] * nf[k]; }
+                double q2 = 0.0;
+                for (int k = 0; k < 3; ++k) q2 += (dv[k] - t * nf[k]) * (dv[k] - t * nf[k]);
+                rad2 = std::max(rad2, q2);
+            }
+            float rad = (float)std::sqrt(rad2);
+            if ((double)rad < std::sqrt(rad2)) rad = std::nextafter(rad, std::numeric_limits<float>::infinity());
+            out.prims[4 * s + 0] = make_float4(r[0], r[1], r[2], m[0]);
+            out.prims[4 * s + 1] = make_float4(r[3], r[4], r[5], m[1]);
+            out.prims[4 * s + 2] = make_float4(r[6], r[7], r[8], m[2]);
+            out.prims[4 * s + 3] = make_float4(nf[0], nf[1], nf[2], rad);
         } else {
             out.prims[s] = make_float4(r[0], r[1], r[2], r[3]);
         }

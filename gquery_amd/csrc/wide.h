@@ -11,7 +11,8 @@
 //      SPairQ<D> 64 B (boxes quantized to 8 bits in 3D / 16 bits in 2D, int16 cones, 8-bit
 //      silhouette extents, reference cones by table index), SPairF<D> 128 B in 3D / 192 B in 2D
 //      (float32; the 2D record carries its reference cones inline).
-//  * primitives in leaf order as float4 records (3D: a, b, c; 2D: a|b), silhouettes in leaf
+//  * primitives in leaf order as float4 records (3D: a, b, c with a circumscribing disk in the
+//    free lanes and a fourth record; 2D: a|b), silhouettes in leaf
 //    order (3D: a, b, n0, n1 with the predicate normals precomputed; 2D: a|b, c).
 // Child references: bit 31 set = leaf, bits 24..30 = count, bits 0..23 = first primitive /
 // silhouette; otherwise the index of the child node / pair record.
@@ -79,6 +80,25 @@ template <int D> struct alignas(64) FNode {
     uint8_t pad[128 - (2 * D * 16 + 20)];
 };
 static_assert(sizeof(FNode<3>) == 128 && sizeof(FNode<2>) == 128, "FNode is two lines");
+
+// Bounding cylinders of the BVH4 children (closest point), one record per node beside QNode /
+// FNode.  Child j's primitives lie in the slab lo[j] <= (p - c).u <= hi[j] along an axis u and
+// within distance rad[j] of the line c + t u, where u is the child's area-weighted mean normal (3D
+// cross(b - a, c - a), 2D (dy, -dx): the reference's normal conventions) and c the centre of the
+// child box as the kernels decode it, 0.5f * (lo + hi) in float32 (so each node format has its own
+// array).  For |u| <= 1, with s = (x - c).u and q = (x - c) - s u,
+// sqrt(max(lo - s, s - hi, 0)^2 + max(|q| - rad, 0)^2) is a lower bound on the distance from x to
+// anything below the child: near the true distance for the thin curved patches of a surface, where
+// the box bound is loose (a query behind a patch is inside its box, one in front sees the box's
+// corners).  Every field is the upper half of a float32 pattern, widened by the kernels with a
+// shift: axis components rounded toward zero (so |u| <= 1), lo down, hi and rad up.  A zero axis
+// (normals that cancel) leaves a sphere about c.
+template <int D> struct alignas(64) CBound {
+    uint16_t ax[D][4];
+    uint16_t lo[4], hi[4], rad[4];
+    uint8_t pad[64 - 8 * D - 24];
+};
+static_assert(sizeof(CBound<3>) == 64 && sizeof(CBound<2>) == 64, "CBound is one line");
 
 // Exact-cull cone of an SNCH child: every predicate normal (oriented consistently) lies within
 // angle acos(cos_t) of `axis`; cos_t/sin_t are rounded so the cone is conservative.  cos_t <= 0
@@ -226,6 +246,7 @@ struct WideHost {
     uint32_t sroot_ref = 0;      // SNCH root pair (0) or leaf ref
     float sroot_lo[3] = {0, 0, 0}, sroot_hi[3] = {0, 0, 0};
     std::vector<uint8_t> nodes, qnodes, snodes, qsnodes;  // raw records
+    std::vector<uint8_t> cbq, cbf;                        // CBound<D> per BVH4 node, quantized / float32 centres
     std::vector<float4> prims;
     std::vector<int32_t> prim_ids;
     std::vector<float4> sils;

@@ -284,8 +284,10 @@ int gq_build_flat_silhouette_edges_device(const float *vertices, int64_t n_verti
  * (float4), 5 primitive ids, 6 silhouettes (float4), 7 silhouette ids, 8 reference cones (3D),
  * 9 big-leaf table, 10 scalars (root_ref, sroot_ref, n_nodes, n_pairs, wide_depth as uint32),
  * 11 reference cones of the 2D quantized pairs, 12 seed cut of the BVH4, 13 seed cut of the SNCH
- * pairs (32-byte entries: box low corner, child reference, box high corner, pad).  Pass
- * out = NULL to get the byte count in *n_bytes. */
+ * pairs (32-byte entries: box low corner, child reference, box high corner, pad), 14 / 15
+ * bounding cylinders of the BVH4 children about the centres of their quantized / float32 boxes
+ * (64 bytes per node, every field the upper 16 bits of a float32: axis [dim][4], lo[4], hi[4],
+ * rad[4]).  Pass out = NULL to get the byte count in *n_bytes. */
 int gq_structure_engine_layout(const gq_structure *s, int which, void *out, int64_t *n_bytes);
 
 /* ---- Kernel timing (replaces the time.time() brackets around dr.eval in the reference demos,

@@ -94,12 +94,18 @@ def test_bvh4_layout(name):
     pts = prims.reshape(n_prims, 4, 4)[:, :3, :3] if D == 3 else prims.reshape(n_prims, 2, 2)
     np.testing.assert_array_equal(pts, V[F[ids]])
     if D == 3:
-        # supporting plane: unit normal and offset, |n.x - o| <= distance for points on the triangle
-        pl = prims.reshape(n_prims, 4, 4)[:, 3, :].astype(np.float64)
-        good = np.linalg.norm(pl[:, :3], axis=1) > 0
+        # circumscribing disk: centre m in the free lanes, |n| <= 1, every vertex within rad of the
+        # line m + t n and within float32 rounding of the plane through m
+        pr = prims.reshape(n_prims, 4, 4).astype(np.float64)
+        m, nrm, rad = pr[:, :3, 3], pr[:, 3, :3], pr[:, 3, 3]
+        assert (np.linalg.norm(nrm, axis=1) <= 1.0).all()
+        np.testing.assert_allclose(m, pts.astype(np.float64).mean(1), rtol=1e-6, atol=1e-7)
         for k in range(3):
-            res = np.abs((pts[:, k].astype(np.float64) * pl[:, :3]).sum(1) - pl[:, 3])
-            assert (res[good] < 1e-5).all()
+            dv = pts[:, k].astype(np.float64) - m
+            h = (dv * nrm).sum(1)
+            lat = np.linalg.norm(dv - h[:, None] * nrm, axis=1)
+            assert (lat <= rad).all()
+            assert (np.abs(h) <= 4e-7 * (1.0 + np.abs(m).sum(1))).all()
     covered = np.zeros(n_prims, np.int64)
     seen = np.zeros(n_nodes, np.int64)
 
@@ -395,3 +401,59 @@ def test_seed_cuts(name):
         assert (covered == 1).all()
         if which == 12 and F.shape[0] > 4096:
             assert len(seeds) > 16   # the cut goes a few levels down on a large tree
+
+
+def halves(raw16):
+    """Upper halves of float32 patterns, widened."""
+    return (raw16.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_bvh4_child_cylinders(name):
+    """Child bounding cylinders (CBound), both arrays: axis length <= 1; every vertex below child j
+    lies in the slab [lo_j, hi_j] along the axis and within rad_j of the axis line through the child
+    box's centre as the kernel computes it (0.5f * (lo + hi) of the decoded quantized box for one
+    array, of the float32 box for the other), checked in float64; tight to the 16-bit rounding."""
+    V, F = SCENES[name]()
+    D = V.shape[1]
+    s = (ext.SNCH if D == 2 else ext.SNCH3D)(V, F)
+    q = s.engine_layout(0).view(qnode_dtype(D))
+    f = s.engine_layout(1).view(fnode_dtype(D))
+    big = s.engine_layout(9).view(np.uint32)
+    n_prims = F.shape[0]
+    prims = s.engine_layout(4).view(np.float32).reshape(-1, 4)
+    pts = prims.reshape(n_prims, 4, 4)[:, :3, :3] if D == 3 else prims.reshape(n_prims, 2, 2)
+
+    def slots(ref):
+        if ref & LEAF:
+            a, c = leaf_range(ref, big)
+            return np.arange(a, a + c)
+        nd = f[ref]
+        return np.concatenate([slots(int(nd["child"][j])) for j in range(int(nd["nchild"]))])
+
+    for which in (14, 15):
+        raw = s.engine_layout(which).view(np.uint8).reshape(-1, 64)
+        assert len(raw) == len(f)
+        h16 = raw[:, :8 * D + 24].copy().view(np.uint16)
+        ax = halves(h16[:, :4 * D]).reshape(-1, D, 4)
+        lo, hi, rad = (halves(h16[:, 4 * D + 4 * k:4 * D + 4 * k + 4]) for k in range(3))
+        for i in range(len(f)):
+            qlo = [decode(q[i]["qlo"][:, j], q[i]["exp"], q[i]["origin"]) for j in range(4)]
+            qhi = [decode(q[i]["qhi"][:, j], q[i]["exp"], q[i]["origin"]) for j in range(4)]
+            for j in range(int(f[i]["nchild"])):
+                if which == 14:
+                    c = (np.float32(0.5) * (qlo[j] + qhi[j]).astype(np.float32)).astype(np.float64)
+                else:
+                    blo = np.array([f[i]["lo"][k][j] for k in range(D)], np.float32)
+                    bhi = np.array([f[i]["hi"][k][j] for k in range(D)], np.float32)
+                    c = (np.float32(0.5) * (blo + bhi)).astype(np.float64)
+                u = ax[i, :, j]
+                assert np.linalg.norm(u) <= 1.0
+                dv = pts[slots(int(f[i]["child"][j]))].reshape(-1, D).astype(np.float64) - c
+                t = dv @ u
+                r = np.linalg.norm(dv - t[:, None] * u[None], axis=1)
+                assert lo[i, j] <= t.min() and t.max() <= hi[i, j] and r.max() <= rad[i, j], (which, i, j)
+                tol = 2.0 ** -7
+                assert t.min() - lo[i, j] <= tol * abs(t.min()) + 1e-30
+                assert hi[i, j] - t.max() <= tol * abs(t.max()) + 1e-30
+                assert rad[i, j] - r.max() <= tol * r.max() + 1e-30
