@@ -550,80 +550,114 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Packet leaf step for triangles, compacted.  A leaf is opened when any lane needs it, but each of
-// its triangles is typically needed by a quarter of the lanes (the union of 64 search balls is much
-// larger than any one of them), and an exact test run for the whole wavefront wastes the rest.  So
-// the lanes first evaluate the cheap lower bounds of every triangle of a chunk (up to kLeafChunk),
-// append their (triangle, lane) pairs to an LDS list in ballot order, and then the wavefront runs the
-// exact test on 64 pairs at a time: lane t takes pair t, reads the owner's query point from LDS and
-// the triangle from memory, and folds (distance, slot) into the owner's LDS minimum (one 64-bit
-// key: distance bits, which order like unsigned integers for d >= 0, then slot, so ties go to the
-// lowest slot exactly as a sequential scan with that rule).  Each lane then merges its minimum into
-// its answer.  All pairs of a chunk are tested against the bound from before the chunk, which only
-// adds tests, never removes one.
-constexpr int kLeafChunk = 8;
+// Packet leaf work for triangles, queued.  The wave opens a child when any lane needs it, but a
+// lane only takes part below a child whose bounds it passed itself (its bit in the child's lane
+// mask, which the wave stack carries with the reference), and with the cylinder bounds few lanes
+// need any one leaf (on the 327,680-triangle icosphere a wavefront of uniform queries visits about
+// 190 leaves while each of its queries needs about 7).  So a leaf visit appends one (triangle, lane)
+// pair per triangle and needing lane to an LDS queue, and the queue is drained a full round at a
+// time: lane t takes pair t, reads the owner's query and its bound at the drain from LDS and the
+// triangle from memory, runs the disk prefilter and the exact test, and folds (distance, slot) into
+// the owner's LDS minimum (one 64-bit key: distance bits, which order like unsigned integers for
+// d >= 0, then slot, so ties go to the lowest slot exactly as a sequential scan with that rule).
+// Each lane then merges its minimum into its answer.  A drain later than the leaf's visit only
+// delays bound updates (more tests, never fewer), and the result does not depend on the order of
+// the tests.  Drain threshold, measured on 1M uniform queries (3D: 327,680-triangle icosphere; 2D:
+// 65,536-segment star): 64 pairs 3.72 / 2.40 ms, 128 3.56 / 2.35, 256 3.57 / 2.30, 448 3.63 / 2.34
+// (draining at once while some lane has no answer yet was no better: 3.61 / 2.31 at 256).
+constexpr int kQueueCap = 512;      // queued pairs at most
+constexpr int kQueueRound = 256;    // a drain starts once this many pairs are queued
 struct PacketLds {
-    float q[64][3];                 // the lanes' query points
-    unsigned long long mn[64];      // per-lane minimum of a chunk: (distance bits << 32) | slot
-    uint16_t pair[kLeafChunk * 64]; // (triangle in chunk << 6) | lane
+    float q[64][3];                 // the lanes' query points (2D: the first two)
+    float b2[64];                   // the lanes' squared pruning bounds at the drain
+    unsigned long long mn[64];      // per-lane minimum of a drain: (distance bits << 32) | slot
+    uint32_t pair[kQueueCap];       // (triangle slot << 6) | lane
 };
 
-__device__ __forceinline__ void cp_leaf_compact(const GP &P, PacketLds &L, uint32_t first, uint32_t count,
-                                                const float *pc, bool live, float slack, float &best,
-                                                int &best_slot, float &bound2) {
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int D>
+__device__ __forceinline__ void cp_drain(const GP &P, PacketLds &L, int &qn, bool live, float slack, float &best,
+                                         int &best_slot, float &bound2) {
     const int lane = threadIdx.x & 63;
-    for (uint32_t c0 = first; c0 < first + count; c0 += kLeafChunk) {
-        const uint32_t cn = min((uint32_t)kLeafChunk, first + count - c0);
-        int total = 0;
-        for (uint32_t j = 0; j < cn; ++j) {
-            const uint32_t s = c0 + j;
-            const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
-            const float4 pl = ldg(P.prims + (4 * s + 3));
-            const bool need = live && tri_may_beat(pc, a, b, c, pl, bound2);
-            const unsigned long long m = __ballot(need);
-            if (need) L.pair[total + lanes_below(m)] = (uint16_t)((j << 6) | (uint32_t)lane);
-            total += __popcll(m);
-        }
-        if (total == 0) continue;
-        L.mn[lane] = ~0ull;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int b0 = 0; b0 < total; b0 += 64) {
-            const int k = b0 + lane;
-            if (k < total) {
-                const uint32_t e = L.pair[k];
-                const uint32_t owner = e & 63u, s = c0 + (e >> 6);
-                const V3 q = v3(L.q[owner][0], L.q[owner][1], L.q[owner][2]);
+    L.b2[lane] = bound2;
+    L.mn[lane] = ~0ull;
+    wave_sync_lds();
+    for (int b0 = 0; b0 < qn; b0 += 64) {
+        const int k = b0 + lane;
+        if (k < qn) {
+            const uint32_t e = L.pair[k];
+            const uint32_t owner = e & 63u, s = e >> 6;
+            const float q[3] = {L.q[owner][0], L.q[owner][1], L.q[owner][2]};
+            if constexpr (D == 3) {
                 const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
-                V3 pt;
-                V2 uv;
-                const float dd = closest_point_triangle(q, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
+                const float4 pl = ldg(P.prims + (4 * s + 3));
+                if (tri_may_beat(q, a, b, c, pl, L.b2[owner])) {
+                    V3 pt;
+                    V2 uv;
+                    const float dd = closest_point_triangle(v3(q[0], q[1], q[2]), v3(a.x, a.y, a.z), v3(b.x, b.y, b.z),
+                                                            v3(c.x, c.y, c.z), pt, uv);
+                    atomicMin(&L.mn[owner], ((unsigned long long)__float_as_uint(dd) << 32) | s);
+                }
+            } else {
+                // a segment's exact test costs about what a prefilter would
+                const float4 ab = ldg(P.prims + (s));
+                V2 pt;
+                const float dd = closest_point_segment(v2(q[0], q[1]), v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
                 atomicMin(&L.mn[owner], ((unsigned long long)__float_as_uint(dd) << 32) | s);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const unsigned long long mk = L.mn[lane];
-        if (live && mk != ~0ull) {
-            const float dd = __uint_as_float((uint32_t)(mk >> 32));
-            const int s = (int)(uint32_t)mk;
-            if (dd < best || (dd == best && s < best_slot)) {
-                best = dd;
-                best_slot = s;
-                bound2 = cp_bound2(dd, slack);
-            }
+    }
+    wave_sync_lds();
+    const unsigned long long mk = L.mn[lane];
+    if (live && mk != ~0ull) {
+        const float dd = __uint_as_float((uint32_t)(mk >> 32));
+        const int s = (int)(uint32_t)mk;
+        if (dd < best || (dd == best && s < best_slot)) {
+            best = dd;
+            best_slot = s;
+            bound2 = cp_bound2(dd, slack);
         }
     }
+    qn = 0;
+}
+
+// Appends the pairs of a leaf's triangles [first, first + count) with the lanes in `act`, draining
+// the queue whenever the next piece would not fit; pairs of one triangle are adjacent.
+template <int D>
+__device__ __forceinline__ void cp_enqueue(const GP &P, PacketLds &L, int &qn, uint32_t first, uint32_t count,
+                                           bool act, bool live, float slack, float &best, int &best_slot,
+                                           float &bound2) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long m = __ballot(act);
+    const int np = __popcll(m);
+    if (np == 0) return;
+    const int rank = lanes_below(m);
+    const uint32_t per = (uint32_t)(kQueueCap / np);   // triangles whose pairs fit an empty queue
+    for (uint32_t c0 = first; c0 < first + count;) {
+        const uint32_t ch = min(per, first + count - c0);
+        if (qn + (int)ch * np > kQueueCap) cp_drain<D>(P, L, qn, live, slack, best, best_slot, bound2);
+        if (act)
+            for (uint32_t t = 0; t < ch; ++t) L.pair[qn + (int)t * np + rank] = ((c0 + t) << 6) | (uint32_t)lane;
+        qn += (int)ch * np;
+        c0 += ch;
+    }
+    if (qn >= kQueueRound) cp_drain<D>(P, L, qn, live, slack, best, best_slot, bound2);
 }
 
 template <int D, bool Q>
 __device__ __forceinline__ void cp_packet(const GP &P, int base) {
     using V = typename VT<D>::T;
     __shared__ uint32_t wstack[kPacketBlock / 64][kPacketDepth];
-    __shared__ PacketLds wlds[D == 3 ? kPacketBlock / 64 : 1];
+    __shared__ unsigned long long wmask[kPacketBlock / 64][kPacketDepth];
+    __shared__ PacketLds wlds[kPacketBlock / 64];
     uint32_t *wst = wstack[threadIdx.x >> 6];
+    unsigned long long *wmk = wmask[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
     const int i = base + threadIdx.x;
     const bool live = i < P.n;
     const uint32_t qi = live ? query_index<D>(P, i) : 0u;
@@ -639,16 +673,18 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
     int best_slot = -1;
     int sp = 0;
     int steps = 0;
-    if constexpr (D == 3) {
+    int qn = 0;   // queued leaf pairs
 #pragma unroll
-        for (int k = 0; k < 3; ++k) wlds[threadIdx.x >> 6].q[threadIdx.x & 63][k] = pc[k];
-    }
+    for (int k = 0; k < D; ++k) wlds[threadIdx.x >> 6].q[lane][k] = pc[k];
+    // `act`: this lane passed the bounds of every node on the way to the current one
+    bool act = live;
     uint32_t cur = uniform(P.root);
     for (;;) {
         if (P.budget > 0 && ++steps > P.budget) {
             // Too much work for one wavefront (queries deep inside a closed surface see most of it
             // at nearly one distance): hand the queries, with the answer so far as the starting
             // bound, to the wave-per-query pass so they spread over many wavefronts.
+            if (qn > 0) cp_drain<D>(P, wlds[threadIdx.x >> 6], qn, live, slack, best, best_slot, bound2);
             defer_wave(P, live, qi, best, best_slot);
             return;
         }
@@ -659,13 +695,13 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
             load_cyl4<D>((const void *)P.cyl, cur, cy);
             float key[4];
             uint32_t ref[4];
-            // children any lane must visit: within the lane's bound of both the child box and the
-            // child's bounding cylinder; the ordering key (distance from the packet centroid) is
-            // computed only when more than one is open (3D: 7.41 vs 7.45 ms; float32 nodes 7.18 vs
-            // 7.32 ms)
-            bool any[4];
+            // children any acting lane must visit: within its bound of both the child box and the
+            // child's bounding cylinder (the cylinder only where some lane's box test passes); the
+            // ordering key (distance from the packet centroid) is computed only when more than one
+            // is open (3D: 7.41 vs 7.45 ms; float32 nodes 7.18 vs 7.32 ms)
+            unsigned long long hm[4];
             int nh = 0;
-            uint32_t only = 0;
+            int only = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 float d2 = 0.0f;
@@ -674,22 +710,28 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
                     const float o = fmaxf(fmaxf(nd.lo[k][j] - pc[k], pc[k] - nd.hi[k][j]), 0.0f);
                     d2 = fmaf(o, o, d2);
                 }
-                float sx;
-                const float lb2 = cyl_lb2<D, Q>(nd, cy, j, pc, sx);
-                any[j] = __ballot((j < nd.n) && (d2 <= bound2) && (lb2 <= bound2)) != 0ull;
-                nh += (int)any[j];
+                const bool box = act && (j < nd.n) && (d2 <= bound2);
+                hm[j] = 0ull;
+                if (__ballot(box) != 0ull) {
+                    float sx;
+                    const float lb2 = cyl_lb2<D, Q>(nd, cy, j, pc, sx);
+                    hm[j] = __ballot(box && (lb2 <= bound2));
+                }
+                nh += (int)(hm[j] != 0ull);
                 ref[j] = nd.child[j];
-                if (any[j]) only = ref[j];
+                if (hm[j] != 0ull) only = j;
             }
             if (nh == 1) {
-                cur = uniform(only);
+                cur = uniform(ref[only]);
+                act = (hm[only] >> lane) & 1ull;
                 continue;
             }
             if (nh > 1) {
+                uint32_t slot[4] = {0u, 1u, 2u, 3u};
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     key[j] = kInf;
-                    if (any[j]) {
+                    if (hm[j] != 0ull) {
                         // box distance of the centroid, or its distance behind the cylinder's slab
                         // (the side opposite the normals) where larger: the slab's near side would
                         // order the tilted patches in front of a convex surface first
@@ -705,11 +747,16 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
                         key[j] = uniformf(fmaxf(c2, behind * behind));
                     }
                 }
-                sort4(key, ref);
+                sort4(key, slot);
 #pragma unroll
                 for (int j = 3; j >= 1; --j)
-                    if (j < nh && sp < kPacketDepth) wst[sp++] = ref[j];
-                cur = uniform(ref[0]);
+                    if (j < nh && sp < kPacketDepth) {
+                        wst[sp] = ref[slot[j]];
+                        wmk[sp] = hm[slot[j]];
+                        ++sp;
+                    }
+                cur = uniform(ref[slot[0]]);
+                act = (hm[slot[0]] >> lane) & 1ull;
                 continue;
             }
         } else {
@@ -717,24 +764,14 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
             leaf_range(cur, (const uint32_t *)P.big, first, count);
             first = uniform(first);
             count = uniform(count);
-            if constexpr (D == 3) {
-                cp_leaf_compact(P, wlds[threadIdx.x >> 6], first, count, pc, live, slack, best, best_slot, bound2);
-            } else {
-                for (uint32_t s = first; s < first + count; ++s) {
-                    const float4 ab = ldg(P.prims + (s));
-                    V2 pt;
-                    const float dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
-                    if (live && (dd < best || (dd == best && (int)s < best_slot))) {
-                        best = dd;
-                        best_slot = (int)s;
-                        bound2 = cp_bound2(dd, slack);
-                    }
-                }
-            }
+            cp_enqueue<D>(P, wlds[threadIdx.x >> 6], qn, first, count, act, live, slack, best, best_slot, bound2);
         }
         if (sp == 0) break;
-        cur = uniform(wst[--sp]);
+        --sp;
+        cur = uniform(wst[sp]);
+        act = (wmk[sp] >> lane) & 1ull;
     }
+    if (qn > 0) cp_drain<D>(P, wlds[threadIdx.x >> 6], qn, live, slack, best, best_slot, bound2);
     if (!live) return;
     cp_record<D>(P, qi, p, best_slot);
 }
