@@ -1869,11 +1869,17 @@ __global__ void __launch_bounds__(kBlock) k_sort_bounds(const EngineParams *J, c
         for (int j = 1; j < kBlock / 64; ++j) v = k < 3 ? min(v, red[j][k]) : max(v, red[j][k]);
         P.part[(tl.y / kBlock) * 6 + k] = v;
     }
+    // this tile's share of the job's cell counters, zeroed for the count pass (a kernel, not a
+    // memset, so batches record into graphs)
+    const int nt = (P.n + kBlock - 1) / kBlock, t = (int)(tl.y / kBlock);
+    const int per = (P.n_cells + nt - 1) / nt;
+    const int c1 = min((t + 1) * per, P.n_cells);
+    for (int c = t * per + threadIdx.x; c < c1; c += kBlock) P.counts[c] = 0u;
 }
 
 constexpr int kScanBlock = 1024;
 
-// (2) one block per job: final bounds from the tile partials, and zeroed cell counters
+// (2) one block per job: final bounds from the tile partials
 __global__ void __launch_bounds__(kScanBlock) k_sort_final(const EngineParams *J) {
     const EngineParams &P = J[blockIdx.x];
     if (!P.perm) return;
@@ -1889,7 +1895,6 @@ __global__ void __launch_bounds__(kScanBlock) k_sort_final(const EngineParams *J
     __shared__ uint32_t red[kScanBlock / 64][6];
     const int w = threadIdx.x >> 6;
     for (int k = 0; k < 6; ++k) red[w][k] = v[k];
-    for (int c = threadIdx.x; c < P.n_cells; c += kScanBlock) P.counts[c] = 0u;
     __syncthreads();
     if (threadIdx.x < 6) {
         const int k = threadIdx.x;
@@ -1922,36 +1927,75 @@ __global__ void __launch_bounds__(kBlock) k_sort_count(const EngineParams *J, co
     atomicAdd(&P.counts[code >> P.cell_shift], 1u);
 }
 
-// (4) one block per job: exclusive prefix sum of the cell counters, in place
-__global__ void __launch_bounds__(kScanBlock) k_sort_scan(const EngineParams *J) {
-    const EngineParams &P = J[blockIdx.x];
-    if (!P.perm) return;
-    const int ncell = P.n_cells;
-    const int per = (ncell + kScanBlock - 1) / kScanBlock;
-    const int c0 = threadIdx.x * per, c1 = min(c0 + per, ncell);
-    uint32_t sum = 0;
-    for (int c = c0; c < c1; ++c) sum += P.counts[c];
-    // block-wide exclusive scan of the per-thread sums
+// (4) exclusive prefix sum of the cell counters, in place, in three passes over chunks of
+// kScanChunk cells: the chunk sums (into P.part, free again once the bounds are final: a job has
+// at most n / 16,384 + 1 chunks and n / 256 * 6 part entries), their exclusive scan (one block per
+// job), then each chunk's scan plus its offset.  Blocks are (chunk, job); a job with fewer chunks
+// leaves the rest idle.
+constexpr int kScanChunk = 4 * kScanBlock;
+// Exclusive scan of one value per thread over a kScanBlock block; *total receives the block sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = sum;
+    uint32_t x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
         if (lane >= o) x += y;
     }
-    __shared__ uint32_t wsum[kScanBlock / 64];
     if (lane == 63) wsum[w] = x;
     __syncthreads();
-    uint32_t before = 0;
-    for (int j = 0; j < w; ++j) before += wsum[j];
-    uint32_t run = before + x - sum;
-    for (int c = c0; c < c1; ++c) {
-        const uint32_t n = P.counts[c];
-        P.counts[c] = run;
-        run += n;
+    uint32_t before = 0, all = 0;
+    for (int j = 0; j < kScanBlock / 64; ++j) {
+        before += j < w ? wsum[j] : 0u;
+        all += wsum[j];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+__global__ void __launch_bounds__(kScanBlock) k_scan_sums(const EngineParams *J) {
+    const EngineParams &P = J[blockIdx.y];
+    const int c0 = (int)blockIdx.x * kScanChunk;
+    if (!P.perm || c0 >= P.n_cells) return;
+    const int c = c0 + 4 * (int)threadIdx.x;
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v += c + k < P.n_cells ? P.counts[c + k] : 0u;
+    uint32_t total;
+    (void)block_exclusive_scan(v, &total);
+    if (threadIdx.x == 0) P.part[blockIdx.x] = total;
+}
+__global__ void __launch_bounds__(kScanBlock) k_scan_top(const EngineParams *J) {
+    const EngineParams &P = J[blockIdx.x];
+    if (!P.perm) return;
+    const int nch = (P.n_cells + kScanChunk - 1) / kScanChunk;
+    uint32_t carry = 0;
+    for (int b = 0; b < nch; b += kScanBlock) {
+        const int i = b + (int)threadIdx.x;
+        const uint32_t v = i < nch ? P.part[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, &total);
+        if (i < nch) P.part[i] = carry + ex;
+        carry += total;
     }
 }
-
+__global__ void __launch_bounds__(kScanBlock) k_scan_apply(const EngineParams *J) {
+    const EngineParams &P = J[blockIdx.y];
+    const int c0 = (int)blockIdx.x * kScanChunk;
+    if (!P.perm || c0 >= P.n_cells) return;
+    const int c = c0 + 4 * (int)threadIdx.x;
+    uint32_t n[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n[k] = c + k < P.n_cells ? P.counts[c + k] : 0u;
+    uint32_t total;
+    uint32_t run = P.part[blockIdx.x] + block_exclusive_scan(n[0] + n[1] + n[2] + n[3], &total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (c + k < P.n_cells) P.counts[c + k] = run;
+        run += n[k];
+    }
+}
 // (5) scatter: each query takes the next slot of its cell
 __global__ void __launch_bounds__(kBlock) k_sort_scatter(const EngineParams *J, const uint2 *tiles) {
     const uint2 tl = tile_at(tiles, blockIdx.x, kBlock);
@@ -2486,15 +2530,20 @@ hipError_t launch_plan(const Plan &plan, uint8_t *base, bool implicit_tiles, hip
             if (di == 0) hipLaunchKernelGGL(k_sort_count<2>, dim3(nt), dim3(kBlock), 0, st, J, t);
             else hipLaunchKernelGGL(k_sort_count<3>, dim3(nt), dim3(kBlock), 0, st, J, t);
         }
-        hipLaunchKernelGGL(k_sort_scan, dim3(nj), dim3(kScanBlock), 0, st, J);
+        int max_cells = 0;
+        for (const EngineParams &P : plan.P) max_cells = std::max(max_cells, P.n_cells);
+        const unsigned chunks = (unsigned)((max_cells + kScanChunk - 1) / kScanChunk);
+        if (chunks > 0) {
+            hipLaunchKernelGGL(k_scan_sums, dim3(chunks, nj), dim3(kScanBlock), 0, st, J);
+            hipLaunchKernelGGL(k_scan_top, dim3(nj), dim3(kScanBlock), 0, st, J);
+            hipLaunchKernelGGL(k_scan_apply, dim3(chunks, nj), dim3(kScanBlock), 0, st, J);
+        }
         for (int di = 0; di < 2; ++di) {
             const unsigned nt = (unsigned)plan.n_sort_tiles[di];
             if (!nt) continue;
             const uint2 *t = tiles ? tiles + plan.sort_tiles_off[di] : nullptr;
             hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(kBlock), 0, st, J, t);
         }
-        int max_cells = 0;
-        for (const EngineParams &P : plan.P) max_cells = std::max(max_cells, P.n_cells);
         if (max_cells > 0)
             hipLaunchKernelGGL(k_sort_refine, dim3((max_cells + kBlock - 1) / kBlock, nj), dim3(kBlock), 0, st, J);
     }
