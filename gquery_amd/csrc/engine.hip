@@ -396,6 +396,69 @@ __device__ __forceinline__ void load_node4(const void *base, uint32_t idx, Node4
     }
 }
 
+// Child box centres and half-extents of a BVH4 node, for the closest-point tests.  Quantized
+// nodes form both from the integer sum and difference of a child's quantized corners (scalar
+// integer work): centre = fma(qlo + qhi, 2^(e - 1), origin), half-extent = (qhi - qlo) * 2^(e - 1);
+// float32 nodes take 0.5f * (lo + hi) and 0.5f * (hi - lo).  The host builds the bounding
+// cylinders about exactly these centres.
+template <int D> struct NodeC4 {
+    float c[D][4], h[D][4];
+    uint32_t child[4];
+    int n;
+};
+template <int D, bool Q>
+__device__ __forceinline__ void load_node_c(const void *base, uint32_t idx, NodeC4<D> &nd) {
+    if constexpr (Q && D == 3) {
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64;
+        const uint4 w0 = ld16(p, 0), w1 = ld16(p, 1), w2 = ld16(p, 2), w3 = ld16(p, 3);
+        const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+        nd.n = (int)(w0.w >> 24);
+        const uint32_t qlo[3] = {w1.x, w1.y, w1.z}, qhi[3] = {w1.w, w2.x, w2.y};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float hs = ldexpf(1.0f, sbyte(w0.w, k) - 1);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t a = (qlo[k] >> (8 * j)) & 0xFFu, b = (qhi[k] >> (8 * j)) & 0xFFu;
+                nd.c[k][j] = fmaf((float)(a + b), hs, org[k]);
+                nd.h[k][j] = (float)(b - a) * hs;
+            }
+        }
+        nd.child[0] = w2.z; nd.child[1] = w2.w; nd.child[2] = w3.x; nd.child[3] = w3.y;
+    } else if constexpr (Q && D == 2) {
+        const uint8_t *p = reinterpret_cast<const uint8_t *>(base) + (size_t)idx * 64;
+        const uint4 w0 = ld16(p, 0), w1 = ld16(p, 1), w2 = ld16(p, 2), w3 = ld16(p, 3);
+        const float org[2] = {__uint_as_float(w0.x), __uint_as_float(w0.y)};
+        nd.n = (int)((w0.z >> 16) & 0xFFu);
+        const uint32_t q[8] = {w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const float hs = ldexpf(1.0f, sbyte(w0.z, k) - 1);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t a = (q[k * 2 + j / 2] >> (16 * (j & 1))) & 0xFFFFu;
+                const uint32_t b = (q[4 + k * 2 + j / 2] >> (16 * (j & 1))) & 0xFFFFu;
+                nd.c[k][j] = fmaf((float)(a + b), hs, org[k]);
+                nd.h[k][j] = (float)(b - a) * hs;
+            }
+        }
+        nd.child[0] = w2.w; nd.child[1] = w3.x; nd.child[2] = w3.y; nd.child[3] = w3.z;
+    } else {
+        Node4<D, Q> b;
+        load_node4<D, Q>(base, idx, b);
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                nd.c[k][j] = 0.5f * (b.lo[k][j] + b.hi[k][j]);
+                nd.h[k][j] = 0.5f * (b.hi[k][j] - b.lo[k][j]);
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nd.child[j] = b.child[j];
+        nd.n = b.n;
+    }
+}
+
 // Bounding cylinders of a BVH4 node's children (CBound<D>, the first 48 bytes of its 64): every
 // field is the upper half of a float32 pattern, widened with a shift (scalar instructions when the
 // record is wave-uniform).
@@ -421,12 +484,15 @@ __device__ __forceinline__ void load_cyl4(const void *base, uint32_t idx, Cyl4<D
 }
 // Squared lower bound on the distance from x to anything below child j: x against the child's
 // bounding cylinder about its box centre c (the same float32 centre the host used), s = (x - c).u
-// the position along the axis (returned for the ordering key).
-template <int D, bool Q>
-__device__ __forceinline__ float cyl_lb2(const Node4<D, Q> &nd, const Cyl4<D> &cy, int j, const float *x, float &s) {
+// the position along the axis (returned for the ordering key).  In 2D the child box is tested as
+// well: a run of a polyline's segments can bend through a wide angle, and the box then prunes what
+// its cylinder cannot (65,536-segment star: 2.25 ms with both tests, 3.22 with the cylinder alone;
+// in 3D the box adds nothing, 3.47 vs 3.53 ms).
+template <int D>
+__device__ __forceinline__ float cyl_lb2(const NodeC4<D> &nd, const Cyl4<D> &cy, int j, const float *x, float &s) {
     float dv[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) dv[k] = x[k] - 0.5f * (nd.lo[k][j] + nd.hi[k][j]);
+    for (int k = 0; k < D; ++k) dv[k] = x[k] - nd.c[k][j];
     s = dv[0] * cy.ax[0][j];
 #pragma unroll
     for (int k = 1; k < D; ++k) s = fmaf(dv[k], cy.ax[k][j], s);
@@ -438,7 +504,17 @@ __device__ __forceinline__ float cyl_lb2(const Node4<D, Q> &nd, const Cyl4<D> &c
     }
     const float sd = fmaxf(fmaxf(cy.lo[j] - s, s - cy.hi[j]), 0.0f);
     const float m = fmaxf(__builtin_amdgcn_sqrtf(q2) - cy.rad[j], 0.0f);
-    return fmaf(sd, sd, m * m);
+    const float lb2 = fmaf(sd, sd, m * m);
+    if constexpr (D == 2) {
+        float d2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float o = fmaxf(fabsf(dv[k]) - nd.h[k][j], 0.0f);
+            d2 = fmaf(o, o, d2);
+        }
+        return fmaxf(lb2, d2);
+    }
+    return lb2;
 }
 
 __device__ __forceinline__ void cswap(float *k, uint32_t *v, int a, int b) {
@@ -689,34 +765,25 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
             return;
         }
         if (!ref_is_leaf(cur)) {
-            Node4<D, Q> nd;
-            load_node4<D, Q>((const void *)P.nodes, cur, nd);
+            NodeC4<D> nd;
+            load_node_c<D, Q>((const void *)P.nodes, cur, nd);
             Cyl4<D> cy;
             load_cyl4<D>((const void *)P.cyl, cur, cy);
             float key[4];
             uint32_t ref[4];
-            // children any acting lane must visit: within its bound of both the child box and the
-            // child's bounding cylinder (the cylinder only where some lane's box test passes); the
-            // ordering key (distance from the packet centroid) is computed only when more than one
-            // is open (3D: 7.41 vs 7.45 ms; float32 nodes 7.18 vs 7.32 ms)
+            // children any acting lane must visit: within its bound of the child's bounding cylinder
+            // (the child box adds nothing to it: the same node visits per query on the
+            // 327,680-triangle icosphere); the ordering key (distance from the packet centroid) is
+            // computed only when more than one is open (3D: 7.41 vs 7.45 ms; float32 nodes 7.18 vs
+            // 7.32 ms)
             unsigned long long hm[4];
             int nh = 0;
             int only = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                float d2 = 0.0f;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float o = fmaxf(fmaxf(nd.lo[k][j] - pc[k], pc[k] - nd.hi[k][j]), 0.0f);
-                    d2 = fmaf(o, o, d2);
-                }
-                const bool box = act && (j < nd.n) && (d2 <= bound2);
-                hm[j] = 0ull;
-                if (__ballot(box) != 0ull) {
-                    float sx;
-                    const float lb2 = cyl_lb2<D, Q>(nd, cy, j, pc, sx);
-                    hm[j] = __ballot(box && (lb2 <= bound2));
-                }
+                float sx;
+                const float lb2 = cyl_lb2<D>(nd, cy, j, pc, sx);
+                hm[j] = __ballot(act && (j < nd.n) && (lb2 <= bound2));
                 nh += (int)(hm[j] != 0ull);
                 ref[j] = nd.child[j];
                 if (hm[j] != 0ull) only = j;
@@ -738,11 +805,11 @@ __device__ __forceinline__ void cp_packet(const GP &P, int base) {
                         float c2 = 0.0f;
 #pragma unroll
                         for (int k = 0; k < D; ++k) {
-                            const float oc = fmaxf(fmaxf(nd.lo[k][j] - cen[k], cen[k] - nd.hi[k][j]), 0.0f);
+                            const float oc = fmaxf(fabsf(cen[k] - nd.c[k][j]) - nd.h[k][j], 0.0f);
                             c2 = fmaf(oc, oc, c2);
                         }
                         float sc;
-                        (void)cyl_lb2<D, Q>(nd, cy, j, cen, sc);
+                        (void)cyl_lb2<D>(nd, cy, j, cen, sc);
                         const float behind = fmaxf(cy.lo[j] - sc, 0.0f);
                         key[j] = uniformf(fmaxf(c2, behind * behind));
                     }
@@ -908,20 +975,14 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
             int cand_slot = -1;
             if (lane < take && __uint_as_float(e.y) <= bound2) {
                 if (!ref_is_leaf(e.x)) {
-                    Node4<D, Q> nd;
-                    load_node4<D, Q>((const void *)P.nodes, e.x, nd);
+                    NodeC4<D> nd;
+                    load_node_c<D, Q>((const void *)P.nodes, e.x, nd);
                     Cyl4<D> cy;
                     load_cyl4<D>((const void *)P.cyl, e.x, cy);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        float d2 = 0.0f;
-#pragma unroll
-                        for (int c = 0; c < D; ++c) {
-                            const float o = fmaxf(fmaxf(nd.lo[c][j] - pc[c], pc[c] - nd.hi[c][j]), 0.0f);
-                            d2 = fmaf(o, o, d2);
-                        }
                         float sx;
-                        const float lb2 = fmaxf(d2, cyl_lb2<D, Q>(nd, cy, j, pc, sx));
+                        const float lb2 = cyl_lb2<D>(nd, cy, j, pc, sx);
                         hit[j] = j < nd.n && lb2 <= bound2;
                         // the entry's key is re-tested against the bound when it is taken
                         ent[j] = make_uint2(nd.child[j], __float_as_uint(lb2));

@@ -255,10 +255,10 @@ void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_l
                     quantize(c.lo[k], c.hi[k], nlo[k], e[k], L, ql, qh);
                     qn.qlo[k][j] = (typename QBits<D>::T)ql;
                     qn.qhi[k][j] = (typename QBits<D>::T)qh;
-                    // box centres exactly as the kernels compute them (no contraction here)
-                    const float dlo = decode(ql, e[k], nlo[k]), dhi = decode(qh, e[k], nlo[k]);
-                    const float sq = dlo + dhi, sf = c.lo[k] + c.hi[k];
-                    cqc[k] = 0.5f * sq;
+                    // box centres exactly as the kernels compute them (no contraction here):
+                    // quantized from the integer corner sum, fma(qlo + qhi, 2^(e - 1), origin)
+                    const float sf = c.lo[k] + c.hi[k];
+                    cqc[k] = std::fmaf((float)(ql + qh), std::ldexp(1.0f, e[k] - 1), nlo[k]);
                     cfc[k] = 0.5f * sf;
                 }
                 double nsum[3];

@@ -85,8 +85,8 @@ static_assert(sizeof(FNode<3>) == 128 && sizeof(FNode<2>) == 128, "FNode is two 
 // FNode.  Child j's primitives lie in the slab lo[j] <= (p - c).u <= hi[j] along an axis u and
 // within distance rad[j] of the line c + t u, where u is the child's area-weighted mean normal (3D
 // cross(b - a, c - a), 2D (dy, -dx): the reference's normal conventions) and c the centre of the
-// child box as the kernels decode it, 0.5f * (lo + hi) in float32 (so each node format has its own
-// array).  For |u| <= 1, with s = (x - c).u and q = (x - c) - s u,
+// child box as the kernels compute it, in float32: fma(qlo + qhi, 2^(e - 1), origin) for the
+// quantized nodes, 0.5f * (lo + hi) for the float32 ones (so each node format has its own array).  For |u| <= 1, with s = (x - c).u and q = (x - c) - s u,
 // sqrt(max(lo - s, s - hi, 0)^2 + max(|q| - rad, 0)^2) is a lower bound on the distance from x to
 // anything below the child: near the true distance for the thin curved patches of a surface, where
 // the box bound is loose (a query behind a patch is inside its box, one in front sees the box's

@@ -412,8 +412,9 @@ def halves(raw16):
 def test_bvh4_child_cylinders(name):
     """Child bounding cylinders (CBound), both arrays: axis length <= 1; every vertex below child j
     lies in the slab [lo_j, hi_j] along the axis and within rad_j of the axis line through the child
-    box's centre as the kernel computes it (0.5f * (lo + hi) of the decoded quantized box for one
-    array, of the float32 box for the other), checked in float64; tight to the 16-bit rounding."""
+    box's centre as the kernel computes it (fma(qlo + qhi, 2^(e - 1), origin) for the quantized
+    nodes, 0.5f * (lo + hi) of the float32 box for the other array), checked in float64; tight to
+    the 16-bit rounding."""
     V, F = SCENES[name]()
     D = V.shape[1]
     s = (ext.SNCH if D == 2 else ext.SNCH3D)(V, F)
@@ -438,11 +439,11 @@ def test_bvh4_child_cylinders(name):
         ax = halves(h16[:, :4 * D]).reshape(-1, D, 4)
         lo, hi, rad = (halves(h16[:, 4 * D + 4 * k:4 * D + 4 * k + 4]) for k in range(3))
         for i in range(len(f)):
-            qlo = [decode(q[i]["qlo"][:, j], q[i]["exp"], q[i]["origin"]) for j in range(4)]
-            qhi = [decode(q[i]["qhi"][:, j], q[i]["exp"], q[i]["origin"]) for j in range(4)]
             for j in range(int(f[i]["nchild"])):
                 if which == 14:
-                    c = (np.float32(0.5) * (qlo[j] + qhi[j]).astype(np.float32)).astype(np.float64)
+                    # fma(qlo + qhi, 2^(e - 1), origin), rounded once to float32
+                    qs = q[i]["qlo"][:, j].astype(np.int64) + q[i]["qhi"][:, j].astype(np.int64)
+                    c = decode(qs, q[i]["exp"].astype(np.int64) - 1, q[i]["origin"]).astype(np.float64)
                 else:
                     blo = np.array([f[i]["lo"][k][j] for k in range(D)], np.float32)
                     bhi = np.array([f[i]["hi"][k][j] for k in range(D)], np.float32)

@@ -130,23 +130,23 @@ def test_absolute_margin_alone_fails_away_from_the_unit_cube():
 
 def _child_paths(s, n_prims, which):
     """For every primitive slot, the (node, child) pairs on its path from the BVH4 root; the child
-    boxes (float32, or decoded from the quantized nodes) and the bounding-cylinder records of array
-    `which` (14: about the quantized box centres, 15: about the float32 ones)."""
+    box centres as the kernels compute them and the bounding-cylinder records of array `which`
+    (14: about the quantized box centres, 15: about the float32 ones)."""
     f = s.engine_layout(1).view(np.uint8).reshape(-1, 128)
     child = f[:, 96:112].copy().view(np.uint32)
     nch = f[:, 112:116].copy().view(np.uint32).reshape(-1)
     if which == 15:
         blo = f[:, 0:48].copy().view(np.float32).reshape(-1, 3, 4)
         bhi = f[:, 48:96].copy().view(np.float32).reshape(-1, 3, 4)
+        cen = (np.float32(0.5) * (blo + bhi).astype(np.float32)).astype(np.float32)
     else:
+        # centre = fma(qlo + qhi, 2^(e - 1), origin), rounded once
         q = s.engine_layout(0).view(np.uint8).reshape(-1, 64)
         org = q[:, 0:12].copy().view(np.float32)
         ex = q[:, 12:15].copy().view(np.int8).astype(np.int64)
-        sc = np.ldexp(1.0, ex)[:, :, None]
-        ql = q[:, 16:28].reshape(-1, 3, 4).astype(np.float64)
-        qh = q[:, 28:40].reshape(-1, 3, 4).astype(np.float64)
-        blo = (ql * sc + org[:, :, None]).astype(np.float32)
-        bhi = (qh * sc + org[:, :, None]).astype(np.float32)
+        hs = np.ldexp(1.0, ex - 1)[:, :, None]
+        qs = q[:, 16:28].reshape(-1, 3, 4).astype(np.float64) + q[:, 28:40].reshape(-1, 3, 4).astype(np.float64)
+        cen = (qs * hs + org[:, :, None]).astype(np.float32)
     big = s.engine_layout(9).view(np.uint32)
     raw = s.engine_layout(which).view(np.uint8).reshape(-1, 64)
     h16 = raw[:, :48].copy().view(np.uint16).astype(np.uint32) << 16
@@ -168,12 +168,11 @@ def _child_paths(s, n_prims, which):
                 walk(ref, here)
 
     walk(0, [])
-    return paths, blo, bhi, ax, lo, hi, rad
+    return paths, cen, ax, lo, hi, rad
 
 
-def cyl_lb2(blo, bhi, ax, lo, hi, rad, x):
-    """cyl_lb2 (engine.hip) in float32, one row per (query, child)."""
-    c = (np.float32(0.5) * (blo + bhi).astype(np.float32)).astype(np.float32)
+def cyl_lb2(c, ax, lo, hi, rad, x):
+    """cyl_lb2 (engine.hip) in float32, one row per (query, child); c the child box centre."""
     dv = (x - c).astype(np.float32)
     sx = (dv[:, 0] * ax[:, 0]).astype(np.float32)
     sx = fma32(dv[:, 1], ax[:, 1], sx)
@@ -208,11 +207,11 @@ def test_cylinder_bounds_hold(offset, scale, which):
     slack = (K_ABS_SLACK * (np.abs(p).sum(1, dtype=np.float32) + s1)).astype(np.float32)
     bb = fma32(d32, np.float32(1.000001), slack)
     bound2 = (bb * bb).astype(np.float32)
-    paths, blo, bhi, ax, lo, hi, rad = _child_paths(s, n_prims, which)
+    paths, cen, ax, lo, hi, rad = _child_paths(s, n_prims, which)
     depth = max(len(q) for q in paths)
     for level in range(depth):
         rows = np.array([i for i in range(len(p)) if len(paths[slot[i]]) > level])
         nj = np.array([paths[slot[i]][level] for i in rows])
         n_, j_ = nj[:, 0], nj[:, 1]
-        lb2 = cyl_lb2(blo[n_, :, j_], bhi[n_, :, j_], ax[n_, :, j_], lo[n_, j_], hi[n_, j_], rad[n_, j_], p[rows])
+        lb2 = cyl_lb2(cen[n_, :, j_], ax[n_, :, j_], lo[n_, j_], hi[n_, j_], rad[n_, j_], p[rows])
         assert (lb2 <= bound2[rows]).all(), (level, np.max(lb2 - bound2[rows]))
