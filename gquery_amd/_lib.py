@@ -24,7 +24,8 @@ METHODS = {"plain": METHOD_PLAIN, "engine_fp32": METHOD_ENGINE_FP32, "engine": M
 
 class SceneOptions(C.Structure):
     _fields_ = [("leaf_prims", C.c_int32), ("packet_budget", C.c_int32), ("traversal", C.c_int32),
-                ("wave_front", C.c_int32), ("sil_pairs", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("wave_front", C.c_int32), ("sil_pairs", C.c_int32), ("ray_leaf_prims", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class Records(C.Structure):

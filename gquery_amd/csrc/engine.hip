@@ -53,28 +53,28 @@ constexpr int kRayStackMax = 96;  // ray stack entries per lane (768 B of privat
 constexpr int kWavePassBlocks = 2048;  // persistent grid of the full-frontier wave-per-query pass
 constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its own L2
 // Packet steps per wavefront before its queries are deferred to the wave-per-query pass, by query
-// kind and dimension (measured optimum with 6-primitive leaves and the two-phase wave pass on 1M
-// uniform queries: 3D closest point 7.73 ms at 1000 vs 8.32 at 800 and 7.92 at 1200; 2D closest
-// point 2.44 ms at 600 vs 2.52 at 750; 3D silhouettes, with near-last appends in the wave pass,
+// kind and dimension (measured optimum on 1M uniform queries: 3D closest point with 24-primitive
+// leaves 2.76 ms at 350 vs 2.79 at 300, 2.86 at 250 and 3.13 at 200; 2D closest point with 16
+// 2.12 ms at 300 vs 2.14 at 400 and 2.28 at 500; 3D silhouettes, with near-last appends in the wave pass,
 // 29.05 ms at 128 vs 29.07 at 64, 29.30 at 250 and 31.47 at 750: the wave pass is the better
 // place for nearly all of their work; 2D silhouettes in packet mode (65,536 segments) 3.28 ms at
 // 750 vs 3.85 at 2,000 and 8.93 at 128: a 2D query has too little work for a whole wavefront).
 // A scene or job option overrides it.
-constexpr int kPacketBudgetCp3 = 1000, kPacketBudgetCp2 = 600, kPacketBudgetSil3 = 128, kPacketBudgetSil2 = 750;
+constexpr int kPacketBudgetCp3 = 350, kPacketBudgetCp2 = 300, kPacketBudgetSil3 = 128, kPacketBudgetSil2 = 750;
 // A launch with few queries leaves most of the GPU idle, and then a packet wavefront's walk -- up to
 // its budget of dependent node fetches -- is the latency of the whole call, while the wave-per-query
 // pass puts 64 lanes on each query.  So below a launch size the default budget drops: measured on
-// the 327,680-triangle icosphere (3D closest point, ms per call, default budget vs 64): 300
-// queries 1.80 vs 0.48, 16,384 2.29 vs 0.76, 65,536 3.09 vs 1.45, 262,144 3.95 vs 3.90, 1,048,576
-// 7.04 vs 13.4; 65,536-segment star (2D closest point): 16,384 1.06 vs 0.45, 65,536 1.20 vs 1.03,
-// 262,144 1.47 vs 3.35; 3D silhouettes at budget 1: 300 0.42 vs 0.25, 65,536 1.53 vs 1.36.
-// When other launches share the device (gq_job.shared; a serving loop's stream slots) the GPU is
-// not idle, and total work decides throughput and queueing: there the full-launch budgets win
+// the 327,680-triangle icosphere with 24-primitive leaves (3D closest point, ms per call, budget 16
+// vs the default): 300 queries 0.57 vs 1.18, 16,384 0.72 vs 1.44, 65,536 1.40 vs 1.70, 262,144
+// 3.60 vs 1.79; 65,536-segment star with 16-primitive leaves: 16,384 0.45 vs 1.05, 65,536 0.99 vs
+// 1.08, 262,144 3.32 vs 1.38; 3D silhouettes at budget 1: 300 0.42 vs 0.25, 65,536 1.53 vs 1.36.
+constexpr int kSmallBudgetCp = 16, kSmallBudgetSil3 = 1;
+constexpr int64_t kSmallLaunchCp3 = 1 << 17, kSmallLaunchCp2 = 1 << 17, kSmallLaunchSil3 = 1 << 18;
+// When other launches share the device (gq_job.shared; a serving loop's stream slots) the GPU is not
+// idle and total work decides throughput and queueing: shared groups are budgeted as full launches
 // (served workload, 2 + 1 stream slots: 140 vs 131 Mq/s saturated, p99 at 100 Mq/s offered 33.4 vs
 // 43.5 ms).
-constexpr int kSmallBudgetCp = 64, kSmallBudgetSil3 = 1;
-constexpr int64_t kSmallLaunchCp3 = 1 << 19, kSmallLaunchCp2 = 1 << 17, kSmallLaunchSil3 = 1 << 18;
-constexpr int64_t kFullLaunch = kSmallLaunchCp3;
+constexpr int64_t kFullLaunch = kSmallLaunchSil3;
 
 // Default packet budget of a launch group of `group_q` queries (no job or scene override).
 int default_budget(int kind, int dim, int64_t group_q) {
@@ -2321,8 +2321,13 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             P.seed = (const Seed *)w.sil_seed;
             P.n_seed = w.n_sil_seed;
         } else {
-            P.nodes = jb.quantized ? w.qnodes : w.nodes;
-            P.cyl = jb.quantized ? w.cbq : w.cbf;
+            if (jb.kind == kIntersect) {
+                P.nodes = jb.quantized ? w.rqnodes : w.rnodes;
+                P.stack_depth = 3 * w.ray_depth + 4;
+            } else {
+                P.nodes = jb.quantized ? w.qnodes : w.nodes;
+                P.cyl = jb.quantized ? w.cbq : w.cbf;
+            }
             P.prims = w.prims;
             P.ids = w.prim_ids;
             P.root = w.root_ref;
@@ -2618,7 +2623,7 @@ hipError_t launch_plan(const Plan &plan, uint8_t *base, bool implicit_tiles, hip
 bool engine_supported(QueryKind kind, const WideView &w) {
     if (kind == kSilhouette) return w.bin_depth + 2 <= kPacketDepth;
     if (kind == kClosestPoint) return 3 * w.wide_depth + 4 <= kPacketDepth;
-    return 3 * w.wide_depth + 4 <= kRayStackMax;
+    return 3 * w.ray_depth + 4 <= kRayStackMax;
 }
 
 hipError_t launch_engine(QueryKind kind, const WideView &w, bool quantized, const QueryArgs &q, hipStream_t st) {

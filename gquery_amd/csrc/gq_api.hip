@@ -262,8 +262,10 @@ int gq_scene_create(const gq_structure *st, gq_scene **out) { return gq_scene_cr
 
 int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_scene **out) {
     if (!st || !out) return fail(GQ_ERR_INVALID, "null argument");
-    const int leaf_prims = (opt && opt->leaf_prims > 0) ? opt->leaf_prims : kDefaultLeafPrims;
-    if (leaf_prims > 16) return fail(GQ_ERR_INVALID, "leaf_prims must be in 1..16");
+    const int leaf_prims = (opt && opt->leaf_prims > 0) ? opt->leaf_prims : default_cp_leaf(st->h.dim);
+    const int ray_leaf = (opt && opt->ray_leaf_prims > 0) ? opt->ray_leaf_prims : kDefaultRayLeaf;
+    if (leaf_prims > kMaxLeafOption || ray_leaf > kMaxLeafOption)
+        return fail(GQ_ERR_INVALID, "leaf_prims and ray_leaf_prims must be in 1..64");
     if (opt && (opt->traversal < 0 || opt->traversal > 2)) return fail(GQ_ERR_INVALID, "traversal must be 0, 1 or 2");
     if (opt && (opt->wave_front < 0 || opt->wave_front > 8192)) return fail(GQ_ERR_INVALID, "wave_front must be 0..8192");
     if (opt && (opt->sil_pairs < 0 || opt->sil_pairs > 1)) return fail(GQ_ERR_INVALID, "sil_pairs must be 0 or 1");
@@ -295,7 +297,7 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     // Engine layouts.
     WideHost wh;
     try {
-        build_wide(h, wh, leaf_prims);
+        build_wide(h, wh, leaf_prims, ray_leaf);
     } catch (const std::bad_alloc &) {
         return cleanup(fail(GQ_ERR_OOM, "host allocation failed while building wide nodes"));
     }
@@ -303,6 +305,7 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     wv.dim = h.dim;
     wv.width = wh.width;
     wv.wide_depth = wh.wide_depth;
+    wv.ray_depth = wh.ray_depth;
     wv.packet_budget = opt ? opt->packet_budget : 0;
     wv.traversal = opt ? opt->traversal : 0;
     wv.wave_front = opt ? opt->wave_front : 0;
@@ -322,6 +325,8 @@ int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_s
     if ((rc = upload(s.get(), wh.qnodes.data(), wh.qnodes.size(), (const uint8_t **)&wv.qnodes))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.cbq.data(), wh.cbq.size(), (const uint8_t **)&wv.cbq))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.cbf.data(), wh.cbf.size(), (const uint8_t **)&wv.cbf))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.rnodes.data(), wh.rnodes.size(), (const uint8_t **)&wv.rnodes))) return cleanup(rc);
+    if ((rc = upload(s.get(), wh.rqnodes.data(), wh.rqnodes.size(), (const uint8_t **)&wv.rqnodes))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.prims.data(), wh.prims.size(), &wv.prims))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.prim_ids.data(), wh.prim_ids.size(), &wv.prim_ids))) return cleanup(rc);
     if ((rc = upload(s.get(), wh.big_leaves.data(), wh.big_leaves.size(), &wv.big_leaves))) return cleanup(rc);
@@ -512,7 +517,7 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     if (!st || !n_bytes) return fail(GQ_ERR_INVALID, "null argument");
     WideHost wh;
     try {
-        build_wide(st->h, wh, kDefaultLeafPrims);
+        build_wide(st->h, wh, default_cp_leaf(st->h.dim), kDefaultRayLeaf);
     } catch (const std::bad_alloc &) {
         return fail(GQ_ERR_OOM, "host allocation failed");
     }
@@ -536,6 +541,8 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     case 13: src = wh.sil_seed.data(); bytes = wh.sil_seed.size() * sizeof(Seed); break;
     case 14: src = wh.cbq.data(); bytes = wh.cbq.size(); break;
     case 15: src = wh.cbf.data(); bytes = wh.cbf.size(); break;
+    case 16: src = wh.rnodes.data(); bytes = wh.rnodes.size(); break;
+    case 17: src = wh.rqnodes.data(); bytes = wh.rqnodes.size(); break;
     default: return fail(GQ_ERR_INVALID, "unknown layout array");
     }
     if (out) {

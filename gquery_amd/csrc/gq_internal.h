@@ -24,17 +24,26 @@ struct PlainView {
 struct RefCone;
 // Engine BVH4 leaf size when the scene options give none (the autotuner's choice on the 3D and 2D
 // closest-point workloads; rays are within 3% of their best at any size from 2 to 6).
-constexpr int kDefaultLeafPrims = 6;
+// Default BVH4 leaf sizes: the closest-point tree (its bounding cylinders and queued leaf drains
+// make large leaves pay; 1M uniform queries, 3D 327,680 triangles: leaf 6 3.47 ms, 16 2.91, 24
+// 2.76, 32 2.79, 48 3.07; 2D 65,536 segments: 6 2.30, 16 2.12, 24 2.20) and the ray tree (a lane
+// tests its leaf serially: 3D 0.49 ms at 6, 0.63 at 16, 0.75 at 24).
+constexpr int kDefaultCpLeaf3 = 24, kDefaultCpLeaf2 = 16, kDefaultRayLeaf = 6;
+inline int default_cp_leaf(int dim) { return dim == 3 ? kDefaultCpLeaf3 : kDefaultCpLeaf2; }
+constexpr int kMaxLeafOption = 64;   // largest gq_scene_options.leaf_prims / ray_leaf_prims
 
 struct WideView {
     int dim;
     int width;                // children per node (4)
-    int wide_depth;           // deepest BVH4 node (root = 0)
+    int wide_depth;           // deepest node of the closest-point BVH4 (root = 0)
+    int ray_depth;            // deepest node of the ray BVH4
     int bin_depth;            // deepest binary node
     int64_t n_nodes, n_snch_nodes, n_prims, n_sils;
     const void *nodes;        // FNode<D>[n_nodes]  float32 BVH4
     const void *qnodes;       // QNode<D>[n_nodes]  quantized BVH4
     const void *cbq, *cbf;    // CBound<D>[n_nodes]  child bounding cylinders about the QNode / FNode box centres
+    const void *rnodes;       // FNode<D>  float32 BVH4 of the rays (its own leaf size)
+    const void *rqnodes;      // QNode<D>  quantized BVH4 of the rays
     const void *snodes;       // SPairF<D>[n_snch_nodes]
     const void *qsnodes;      // SPairQ<D>[n_snch_nodes]
     const float4 *prims;      // 3D: 3 float4 per triangle (a, b, c); 2D: 1 float4 per segment (a|b)

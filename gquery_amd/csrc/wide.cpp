@@ -164,8 +164,13 @@ void child_cylinder(const HostStructure &h, uint32_t pb, uint32_t pc, const uint
     cb.rad[j] = half_bound(std::sqrt(rad2), 1);
 }
 
+// Collapses the binary tree into 4-wide nodes with leaves of at most max_leaf primitives.  The
+// closest-point tree (ray = false) fills nodes / qnodes / wide_depth and the bounding cylinders;
+// the rays' tree fills rnodes / rqnodes / ray_depth.
 template <int D>
-void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_leaf, WideHost &out, LeafTable &lt) {
+void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_leaf, WideHost &out, LeafTable &lt,
+                bool ray) {
+    int &depth_out = ray ? out.ray_depth : out.wide_depth;
     struct Tmp {
         int n = 0;
         int bin[4];
@@ -180,7 +185,7 @@ void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_l
     };
     auto is_leafish = [&](int i) { return b[i].leaf || (int)b[i].pcount <= max_leaf; };
     std::function<uint32_t(int, int)> make = [&](int bi, int depth) -> uint32_t {
-        out.wide_depth = std::max(out.wide_depth, depth);
+        depth_out = std::max(depth_out, depth);
         uint32_t idx = (uint32_t)tmp.size();
         tmp.emplace_back();
         int list[4], n = 0;
@@ -212,11 +217,14 @@ void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_l
     };
     make(0, 0);
     const size_t n = tmp.size();
-    out.n_nodes = n;
-    out.nodes.assign(n * sizeof(FNode<D>), 0);
-    out.qnodes.assign(n * sizeof(QNode<D>), 0);
-    out.cbq.assign(n * sizeof(CBound<D>), 0);
-    out.cbf.assign(n * sizeof(CBound<D>), 0);
+    std::vector<uint8_t> &fnodes = ray ? out.rnodes : out.nodes, &qnodes = ray ? out.rqnodes : out.qnodes;
+    (ray ? out.n_ray_nodes : out.n_nodes) = n;
+    fnodes.assign(n * sizeof(FNode<D>), 0);
+    qnodes.assign(n * sizeof(QNode<D>), 0);
+    if (!ray) {
+        out.cbq.assign(n * sizeof(CBound<D>), 0);
+        out.cbf.assign(n * sizeof(CBound<D>), 0);
+    }
     constexpr int L = QBits<D>::levels;
     for (size_t i = 0; i < n; ++i) {
         const Tmp &t = tmp[i];
@@ -261,6 +269,7 @@ void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_l
                     cqc[k] = std::fmaf((float)(ql + qh), std::ldexp(1.0f, e[k] - 1), nlo[k]);
                     cfc[k] = 0.5f * sf;
                 }
+                if (ray) continue;
                 double nsum[3];
                 normal_sum<D>(h, c.pbegin, c.pcount, nsum);
                 const double len = std::sqrt(nsum[0] * nsum[0] + nsum[1] * nsum[1] + nsum[2] * nsum[2]);
@@ -280,10 +289,12 @@ void build_bvh4(const HostStructure &h, const std::vector<BinNode> &b, int max_l
                 fn.child[j] = qn.child[j] = 0;
             }
         }
-        std::memcpy(out.nodes.data() + i * sizeof(FNode<D>), &fn, sizeof fn);
-        std::memcpy(out.qnodes.data() + i * sizeof(QNode<D>), &qn, sizeof qn);
-        std::memcpy(out.cbq.data() + i * sizeof(CBound<D>), &cq, sizeof cq);
-        std::memcpy(out.cbf.data() + i * sizeof(CBound<D>), &cf, sizeof cf);
+        std::memcpy(fnodes.data() + i * sizeof(FNode<D>), &fn, sizeof fn);
+        std::memcpy(qnodes.data() + i * sizeof(QNode<D>), &qn, sizeof qn);
+        if (!ray) {
+            std::memcpy(out.cbq.data() + i * sizeof(CBound<D>), &cq, sizeof cq);
+            std::memcpy(out.cbf.data() + i * sizeof(CBound<D>), &cf, sizeof cf);
+        }
     }
 }
 
@@ -756,10 +767,11 @@ template <int D> void build_seeds(WideHost &out) {
     });
 }
 
-template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_leaf) {
+template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_leaf, int ray_leaf) {
     std::vector<BinNode> b = read_bvh<D>(h);
     LeafTable lt;
-    build_bvh4<D>(h, b, max_leaf, out, lt);
+    build_bvh4<D>(h, b, max_leaf, out, lt, false);
+    build_bvh4<D>(h, b, ray_leaf, out, lt, true);
     out.root_ref = 0;
     for (int k = 0; k < D; ++k) { out.scene_lo[k] = b[0].lo[k]; out.scene_hi[k] = b[0].hi[k]; }
     // primitives in leaf order
@@ -817,11 +829,12 @@ template <int D> void build_impl(const HostStructure &h, WideHost &out, int max_
 
 } // namespace
 
-void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims) {
+void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims, int ray_leaf_prims) {
     out = WideHost();
-    max_leaf_prims = std::clamp(max_leaf_prims, 1, 16);
-    if (h.dim == 2) build_impl<2>(h, out, max_leaf_prims);
-    else build_impl<3>(h, out, max_leaf_prims);
+    max_leaf_prims = std::clamp(max_leaf_prims, 1, kMaxLeafOption);
+    ray_leaf_prims = std::clamp(ray_leaf_prims, 1, kMaxLeafOption);
+    if (h.dim == 2) build_impl<2>(h, out, max_leaf_prims, ray_leaf_prims);
+    else build_impl<3>(h, out, max_leaf_prims, ray_leaf_prims);
 }
 
 } // namespace gqa

@@ -247,6 +247,11 @@ struct WideHost {
     float sroot_lo[3] = {0, 0, 0}, sroot_hi[3] = {0, 0, 0};
     std::vector<uint8_t> nodes, qnodes, snodes, qsnodes;  // raw records
     std::vector<uint8_t> cbq, cbf;                        // CBound<D> per BVH4 node, quantized / float32 centres
+    // the rays' BVH4: the same binary tree collapsed with its own leaf size (leaf references index
+    // the same primitive slots)
+    int ray_depth = 0;
+    size_t n_ray_nodes = 0;
+    std::vector<uint8_t> rnodes, rqnodes;
     std::vector<float4> prims;
     std::vector<int32_t> prim_ids;
     std::vector<float4> sils;
@@ -257,7 +262,8 @@ struct WideHost {
     std::vector<Seed> bvh_seed, sil_seed;   // seed cuts of the BVH4 and of the SNCH pairs
 };
 
-void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims);
+// max_leaf_prims: leaf size of the closest-point BVH4; ray_leaf_prims: of the rays' BVH4.
+void build_wide(const HostStructure &h, WideHost &out, int max_leaf_prims, int ray_leaf_prims);
 
 // One query batch for the engine: a scene's device layout, the query kind and node format, the
 // caller's arrays, and per-batch overrides (0 = the scene's setting).

@@ -190,7 +190,7 @@ class Scene(_DeviceQueries):
     """One polyline or mesh on the device: reference-layout BVH/SNCH and the engine layouts."""
 
     def __init__(self, structure, device=None, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0,
-                 wave_front: int = 0, sil_pairs: int = 0):
+                 wave_front: int = 0, sil_pairs: int = 0, ray_leaf_prims: int = 0):
         self.device = _device(device)
         self.dim = structure.info()["dim"]
         self.has_snch = bool(structure.info()["is_snch"])
@@ -199,7 +199,7 @@ class Scene(_DeviceQueries):
         with torch.cuda.device(self.device):
             opt = _lib.SceneOptions(leaf_prims=int(leaf_prims), packet_budget=int(packet_budget),
                                     traversal=int(traversal), wave_front=int(wave_front),
-                                    sil_pairs=int(sil_pairs))
+                                    sil_pairs=int(sil_pairs), ray_leaf_prims=int(ray_leaf_prims))
             _lib.check(_lib.lib().gq_scene_create_ex(structure.handle, C.byref(opt), C.byref(h)), "gq_scene_create")
         self._h = h
         # held by the object: module globals may already be gone when it is collected at exit

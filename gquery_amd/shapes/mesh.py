@@ -12,7 +12,7 @@ from ..scene import INF, BruteForce, Scene
 class Mesh:
     def __init__(self, vertices, indices, device=None, max_prims_in_node: int = 10, leaf_prims: int = 0,
                  packet_budget: int = 0, traversal: int = 0, wave_front: int = 0, sil_pairs: int = 0,
-                 structure=None):
+                 structure=None, ray_leaf_prims: int = 0):
         # structure: an ext.SNCH3D already built or loaded from a scene file (then vertices and
         # indices may be None)
         self.snch = structure if structure is not None else ext.SNCH3D(vertices, indices, max_prims_in_node)
@@ -21,7 +21,8 @@ class Mesh:
         self.bvh = self.snch
         self.vertices = self.snch._vertices
         self.indices = self.snch._indices
-        self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal, wave_front, sil_pairs)
+        self.scene = Scene(self.snch, device, leaf_prims, packet_budget, traversal, wave_front, sil_pairs,
+                           ray_leaf_prims)
         self._device = self.scene.device
         self._bf = None
         self._normals = None
@@ -93,9 +94,10 @@ class Mesh:
         return self.scene.sample_boundary(x, R, sampler)
 
     def set_options(self, leaf_prims: int = 0, packet_budget: int = 0, traversal: int = 0, wave_front: int = 0,
-                    sil_pairs: int = 0):
+                    sil_pairs: int = 0, ray_leaf_prims: int = 0):
         """Rebuilds the device scene with other engine options (the autotuner's choice)."""
-        self.scene = Scene(self.snch, self._device, leaf_prims, packet_budget, traversal, wave_front, sil_pairs)
+        self.scene = Scene(self.snch, self._device, leaf_prims, packet_budget, traversal, wave_front, sil_pairs,
+                           ray_leaf_prims)
 
     def save(self, path) -> None:
         """Writes the mesh's trees and geometry to a scene file (Mesh.load reads it back

@@ -1,7 +1,8 @@
 """Autotuner: picks the engine's per-scene settings by timing candidates on the device.
 
 Knobs (all exact: every candidate returns the same records):
-  * leaf_prims     -- primitives per BVH4 leaf (node width of the leaf level: 2, 3, 4 or 6);
+  * leaf_prims     -- primitives per leaf of the closest-point BVH4 (8, 16, 24 or 32);
+  * ray_leaf_prims -- primitives per leaf of the rays' BVH4 (4, 6 or 8);
   * packet_budget  -- packet steps a wavefront takes before its queries move to the
                       wave-per-query pass (how much work runs wave-per-query vs 64-lane packets);
   * traversal      -- silhouettes: packet + wave-per-query pass, or lane per query.
@@ -9,7 +10,7 @@ The traversal stack depth is not a knob: it is fixed by the tree (the engine siz
 and frontiers from the scene's depth at launch).
 
     best = tune_shape(V, F, queries={"cp": x_cp, "sil": x_sil})
-    mesh = Mesh(V, F, **best["options"])        # leaf_prims, packet_budget, traversal
+    mesh = Mesh(V, F, **best["options"])        # leaf_prims, ray_leaf_prims, packet_budget, traversal
 
 Timing uses a sample of the caller's queries (or uniform points in the scene box enlarged by
 25%), median of a few runs per candidate after a warm-up.
@@ -22,8 +23,9 @@ import time
 import numpy as np
 import torch
 
-LEAVES = (2, 3, 4, 6)
-BUDGETS = (128, 500, 1000, 1500, 2000)
+LEAVES = (8, 16, 24, 32)
+RAY_LEAVES = (4, 6, 8)
+BUDGETS = (128, 350, 600, 1000)
 TRAVERSALS = (1, 2)
 
 
@@ -62,7 +64,8 @@ def sample_queries(V, n, seed=0):
 
 
 def tune_shape(V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1 << 18,
-               device="cuda:0", leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, log=None):
+               device="cuda:0", leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, log=None,
+               ray_leaves=RAY_LEAVES):
     """Returns {"options": {...}, "ms": {kind: ms}, "trials": [...]} for the polyline (V: n x 2) or
     mesh (V: n x 3) with primitives F.
 
@@ -78,6 +81,17 @@ def tune_shape(V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1
         x, v = torch.from_numpy(x).to(device), torch.from_numpy(v).to(device)
         queries = {k: (x, v) for k in kinds}
     trials = []
+    # rays: their own tree's leaf size (the packet budget and traversal do not apply)
+    ray_best = None
+    if "ray" in kinds:
+        for rl in ray_leaves:
+            scene = Scene(structure, device, ray_leaf_prims=rl)
+            x, v = queries["ray"]
+            t_ms = _time(lambda: _run(scene, "ray", x, v))
+            trials.append({"ray_leaf_prims": rl, "ms": {"ray": t_ms}})
+            if ray_best is None or t_ms < ray_best[1]:
+                ray_best = (rl, t_ms)
+            del scene
     for leaf in leaves:
         for budget, trav in itertools.product(budgets, traversals):
             if trav == 2 and budget != budgets[0]:
@@ -85,7 +99,7 @@ def tune_shape(V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1
             scene = Scene(structure, device, leaf, budget, trav)
             ms = {}
             for k in kinds:
-                if k != "sil" and trav == 2:
+                if k == "ray" or (k != "sil" and trav == 2):
                     continue
                 x, v = queries[k]
                 ms[k] = _time(lambda: _run(scene, k, x, v))
@@ -93,33 +107,37 @@ def tune_shape(V, F, queries=None, kinds=("cp", "ray", "sil"), n_sample: int = 1
             if log:
                 log(f"tune leaf={leaf} budget={budget} traversal={trav}: {ms}")
             del scene
-    # leaf size and budget: minimise the summed closest-point + ray time; silhouettes pick their
+    # closest-point leaf size and budget: minimise the closest-point time; silhouettes pick their
     # traversal (and budget) on the chosen leaf size
-    fast = [k for k in kinds if k != "sil"]
-    packet = [t for t in trials if t["traversal"] == 1]
-    if fast:
-        best = min(packet, key=lambda t: sum(t["ms"][k] for k in fast))
+    packet = [t for t in trials if t.get("traversal") == 1]
+    if "cp" in kinds:
+        best = min(packet, key=lambda t: t["ms"]["cp"])
     else:
         best = min(packet, key=lambda t: t["ms"].get("sil", 0.0))
     opts = {"leaf_prims": best["leaf_prims"], "packet_budget": best["packet_budget"], "traversal": 1}
-    ms = {k: best["ms"][k] for k in fast}
+    ms = {"cp": best["ms"]["cp"]} if "cp" in kinds else {}
+    if ray_best is not None:
+        opts["ray_leaf_prims"] = ray_best[0]
+        ms["ray"] = ray_best[1]
     if "sil" in kinds:
-        same_leaf = [t for t in trials if t["leaf_prims"] == best["leaf_prims"] and "sil" in t["ms"]]
+        same_leaf = [t for t in trials if t.get("leaf_prims") == best["leaf_prims"] and "sil" in t["ms"]]
         s = min(same_leaf, key=lambda t: t["ms"]["sil"])
         opts["traversal"] = s["traversal"]
         ms["sil"] = s["ms"]["sil"]
-        if not fast:
+        if "cp" not in kinds:
             opts["packet_budget"] = s["packet_budget"]
     return {"options": opts, "ms": ms, "trials": trials}
 
 
 
-def tune_jobs(jobs, leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, method: str = "engine", log=None):
+def tune_jobs(jobs, leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, method: str = "engine", log=None,
+              ray_leaves=RAY_LEAVES):
     """Options for a batch of jobs [(shape, kind, x, v), ...] (Mesh / Polyline shapes), timed as they
     will run: jobs are grouped by (dimension, query kind, primitive count within a factor of 4), and
     for every leaf size each group's jobs run as one gq_batch_enqueue batch with every packet
-    budget / traversal candidate as per-job overrides (one device scene per shape and leaf size).
-    Returns one options dict per job (leaf_prims, packet_budget, traversal)."""
+    budget / traversal candidate as per-job overrides (one device scene per shape and leaf size; ray
+    groups take their leaf size from ray_leaves).  Returns one options dict per job (leaf_prims or
+    ray_leaf_prims, packet_budget, traversal)."""
     from .batch import Batch
     from .scene import Scene
     groups = {}
@@ -128,10 +146,11 @@ def tune_jobs(jobs, leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, metho
         key = (shape.scene.dim, kind, int(np.log2(max(n_prims, 1)) // 2))
         groups.setdefault(key, []).append(k)
     best = {key: (float("inf"), None) for key in groups}
-    for leaf in leaves:
+    for i in range(max(len(leaves), len(ray_leaves))):
+        leaf, ray_leaf = leaves[min(i, len(leaves) - 1)], ray_leaves[min(i, len(ray_leaves) - 1)]
         scenes = {}
         for k, (shape, _, _, _) in enumerate(jobs):
-            scenes[k] = Scene(shape.snch, shape.scene.device, leaf)
+            scenes[k] = Scene(shape.snch, shape.scene.device, leaf, ray_leaf_prims=ray_leaf)
         for key, ks in groups.items():
             kind = key[1]
             for budget, trav in itertools.product(budgets, traversals):
@@ -141,7 +160,8 @@ def tune_jobs(jobs, leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, metho
                                 traversal=trav) for k in ks], method=method)
                 ms = _time(b.run)
                 if ms < best[key][0]:
-                    best[key] = (ms, {"leaf_prims": leaf, "packet_budget": budget, "traversal": trav})
+                    lk = {"ray_leaf_prims": ray_leaf} if kind == "ray" else {"leaf_prims": leaf}
+                    best[key] = (ms, {**lk, "packet_budget": budget, "traversal": trav})
                 del b
         del scenes
     out = [None] * len(jobs)

@@ -119,10 +119,11 @@ int gq_scene_destroy(gq_scene *scene);
 
 /* Engine layout options (no counterpart in the reference; chosen by the autotuner). */
 typedef struct gq_scene_options {
-    int32_t leaf_prims;    /* max primitives per engine BVH4 leaf, 1..16; 0 = default (6) */
+    int32_t leaf_prims;    /* max primitives per leaf of the closest-point BVH4, 1..64; 0 = default
+                              (24 in 3D, 16 in 2D) */
     int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
-                              move to the wave-per-query pass; 0 = default (closest point 1000 in
-                              3D / 600 in 2D, silhouettes 128 in 3D / 750 in 2D), < 0 = never */
+                              move to the wave-per-query pass; 0 = default (closest point 350 in
+                              3D / 300 in 2D, silhouettes 128 in 3D / 750 in 2D), < 0 = never */
     int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D, packet + wave
                               pass in 3D), 1 = packet + wave pass, 2 = lane per query */
     int32_t wave_front;    /* LDS frontier entries of the first wave-per-query pass, 1..8192;
@@ -133,7 +134,9 @@ typedef struct gq_scene_options {
                               128 B per pair in 3D, 192 B in 2D with the reference cones inline),
                               1 = quantized (64 B per pair: 8-bit boxes in 3D, 16-bit in 2D, int16
                               cones, 2D reference cones from a table).  Results do not change */
-    int32_t reserved[3];
+    int32_t ray_leaf_prims; /* max primitives per leaf of the rays' BVH4 (the same binary tree
+                              collapsed separately), 1..64; 0 = default (6) */
+    int32_t reserved[2];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
 /* info[0..7] = dim, has_snch, device bytes, n_nodes, n_wide_nodes, n_prims, n_silhouettes, 0 */
@@ -287,7 +290,9 @@ int gq_build_flat_silhouette_edges_device(const float *vertices, int64_t n_verti
  * pairs (32-byte entries: box low corner, child reference, box high corner, pad), 14 / 15
  * bounding cylinders of the BVH4 children about the centres of their quantized / float32 boxes
  * (64 bytes per node, every field the upper 16 bits of a float32: axis [dim][4], lo[4], hi[4],
- * rad[4]).  Pass out = NULL to get the byte count in *n_bytes. */
+ * rad[4]), 16 / 17 float32 / quantized BVH4 nodes of the rays (ray leaf size).  Arrays 0, 1,
+ * 4, 5, 9, 12, 14, 15 are built with the default closest-point leaf size.  Pass out = NULL to get
+ * the byte count in *n_bytes. */
 int gq_structure_engine_layout(const gq_structure *s, int which, void *out, int64_t *n_bytes);
 
 /* ---- Kernel timing (replaces the time.time() brackets around dr.eval in the reference demos,

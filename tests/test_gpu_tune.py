@@ -18,10 +18,12 @@ def test_tuned_options_give_the_default_records(gpu, dim):
     x = torch.from_numpy(synth.uniform_points(20000, dim, seed=31)).cuda()
     v = torch.from_numpy(synth.uniform_directions(20000, dim, seed=32)).cuda()
     res = tune.tune_shape(V, F, queries={k: (x, v) for k in ("cp", "ray", "sil")},
-                          leaves=(2, 4), budgets=(1, 1000), traversals=(1, 2))
+                          leaves=(2, 16), budgets=(1, 1000), traversals=(1, 2), ray_leaves=(4, 6))
     opt = res["options"]
-    assert opt["leaf_prims"] in (2, 4) and opt["packet_budget"] in (1, 1000) and opt["traversal"] in (1, 2)
-    assert len(res["trials"]) == 2 * 3 and set(res["ms"]) == {"cp", "ray", "sil"}
+    assert opt["leaf_prims"] in (2, 16) and opt["packet_budget"] in (1, 1000) and opt["traversal"] in (1, 2)
+    assert opt["ray_leaf_prims"] in (4, 6)
+    # two ray-tree trials, then per closest-point leaf size two budgets plus lane-per-query silhouettes
+    assert len(res["trials"]) == 2 + 2 * 3 and set(res["ms"]) == {"cp", "ray", "sil"}
     cls = Polyline if dim == 2 else Mesh
     base, tuned = cls(V, F, device="cuda:0"), cls(V, F, device="cuda:0", **opt)
     for q in ("cp", "ray", "sil"):
@@ -44,7 +46,7 @@ def test_tune_jobs_groups_and_keeps_results(gpu):
         dim = shape.scene.dim
         jobs.append((shape, kind, torch.from_numpy(synth.uniform_points(9000, dim, seed=70 + k)).cuda(), None))
     ref = [j[0].closest_point_bvh(j[2]) if j[1] == "cp" else j[0].closest_silhouette_snch(j[2]) for j in jobs]
-    opts = tune.tune_jobs(jobs, leaves=(2, 6), budgets=(500, 1000), traversals=(1, 2))
+    opts = tune.tune_jobs(jobs, leaves=(2, 16), budgets=(500, 1000), traversals=(1, 2), ray_leaves=(4, 6))
     assert opts[0] == opts[1]                       # same group: same dimension, kind and size class
     for (shape, _, _, _), o in zip(jobs, opts):
         shape.set_options(**o)
