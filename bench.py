@@ -374,8 +374,13 @@ def batched(args):
         from gquery_amd import tune
         t1 = time.perf_counter()
         tuned = tune.tune_jobs(jobs, log=log)
-        for (shape, _, _, _), opt in zip(jobs, tuned):
-            shape.set_options(**opt)
+
+        def apply(a):
+            torch.cuda.set_device(0)
+            a[0][0].set_options(**a[1])
+
+        with cf.ThreadPoolExecutor(max_workers=8) as ex:
+            list(ex.map(apply, zip(jobs, tuned)))
         log(f"autotuned {len(set(map(str, tuned)))} option sets in {time.perf_counter() - t1:.0f}s")
     stream = torch.cuda.current_stream()
 

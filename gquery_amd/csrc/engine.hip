@@ -76,14 +76,57 @@ constexpr int64_t kSmallLaunchCp3 = 1 << 17, kSmallLaunchCp2 = 1 << 17, kSmallLa
 // 43.5 ms).
 constexpr int64_t kFullLaunch = kSmallLaunchSil3;
 
-// Default packet budget of a launch group of `group_q` queries (no job or scene override).
-int default_budget(int kind, int dim, int64_t group_q) {
+// The budgets above are the optimum for scenes of about 65k-330k primitives.  The best budget grows
+// with the tree (a packet needs more steps to finish) and with the query density rho = queries per
+// primitive (denser packets share more of their walk).  Measured with one-job launches of uniform
+// queries (displaced icospheres s = 3..8, stars of 1k-1M segments; 262k, 1M and 4M queries; ms):
+//   3D closest point: s <= 5 (<= 20k triangles) 128 beats 350 by 5-25% (s5, 1M: 1.14 vs 1.21);
+//     s8 (1.3M triangles) 1,000 beats 350 (1M: 6.35 vs 8.56, 4M: 14.2 vs 17.0).
+//   2D closest point: <= 16k segments 128 beats 300 (16k, 262k queries: 0.68 vs 0.93); from 262k
+//     segments with rho >= 1, 1,000 (512k segments, 1M: 6.56 vs 10.8; 1M segments, 4M: 23.7 vs 49.5).
+//   3D silhouettes: with rho >= 128 nearly every packet finishes and budget 3,000 wins (s4, 1M:
+//     4.96 vs 11.95; s5, 4M: 26.3 vs 46.8); for 24 <= rho < 128 lane per query wins (s4 at 262k
+//     queries 2.51 vs 3.34, s5 at 1M 9.35 vs 11.98, s6 at 4M 42.9 vs 48.2); below that 128.
+//   2D silhouettes: lane per query keeps its stack in LDS, which a deep tree makes costly: from
+//     393k segments packets win (512k segments, 1M queries: 8.86 at 3,000 vs 11.26 per lane; 1M
+//     segments, 262k queries: 8.11 at 64 vs 27.4), with 3,000 at rho >= 1 and 128 below.
+constexpr int kLongBudget = 3000;
+constexpr int kPacketBudgetSmallTree = 128;
+constexpr int kPacketBudgetLargeTree = 1000;
+constexpr int64_t kCp3SmallTree = 1 << 16, kCp3LargeTree = 1 << 19;
+constexpr int64_t kCp2SmallTree = 1 << 14, kCp2LargeTree = 1 << 18;
+constexpr int64_t kSil2PacketTree = 3 << 17;
+constexpr double kSil3LaneRho = 24.0, kSil3DenseRho = 128.0;
+
+// Default packet budget of a job of n queries on a scene of `prims` primitives, in a launch group
+// of `group_q` queries (no job or scene override).
+int default_budget(int kind, int dim, int64_t group_q, int64_t n, int64_t prims) {
+    const double rho = (double)n / (double)std::max<int64_t>(prims, 1);
     if (kind == kSilhouette) {
-        if (dim == 3) return group_q < kSmallLaunchSil3 ? kSmallBudgetSil3 : kPacketBudgetSil3;
+        if (dim == 3) {
+            if (group_q < kSmallLaunchSil3) return kSmallBudgetSil3;
+            return rho >= kSil3DenseRho ? kLongBudget : kPacketBudgetSil3;
+        }
+        if (prims >= kSil2PacketTree) return rho >= 1.0 ? kLongBudget : kPacketBudgetSmallTree;
         return kPacketBudgetSil2;
     }
-    if (dim == 3) return group_q < kSmallLaunchCp3 ? kSmallBudgetCp : kPacketBudgetCp3;
-    return group_q < kSmallLaunchCp2 ? kSmallBudgetCp : kPacketBudgetCp2;
+    if (dim == 3) {
+        if (group_q < kSmallLaunchCp3) return kSmallBudgetCp;
+        return prims < kCp3SmallTree ? kPacketBudgetSmallTree
+             : prims < kCp3LargeTree ? kPacketBudgetCp3 : kPacketBudgetLargeTree;
+    }
+    if (group_q < kSmallLaunchCp2) return kSmallBudgetCp;
+    if (prims <= kCp2SmallTree) return kPacketBudgetSmallTree;
+    return prims >= kCp2LargeTree && rho >= 1.0 ? kPacketBudgetLargeTree : kPacketBudgetCp2;
+}
+
+// Default silhouette traversal of a job (no job or scene override): lane per query for 2D scenes
+// below kSil2PacketTree primitives and for full-size 3D launches at a middle query density,
+// packets otherwise (measurements above).
+bool default_sil_lane(int dim, int64_t n, int64_t prims) {
+    if (dim == 2) return prims < kSil2PacketTree;
+    const double rho = (double)n / (double)std::max<int64_t>(prims, 1);
+    return n >= kSmallLaunchSil3 && rho >= kSil3LaneRho && rho < kSil3DenseRho;
 }
 constexpr int kSortMin = 8192;   // smaller batches are traversed in input order
 constexpr float kInf = __builtin_inff();
@@ -2239,7 +2282,8 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
         const EngineJob &jb = jobs[j];
         const WideView &w = *jb.w;
         const bool sil_lane = jb.kind == kSilhouette &&
-                              (jb.traversal == 2 || (jb.traversal == 0 && w.dim == 2)) && w.bin_depth + 2 <= kRayStackMax;
+                              (jb.traversal == 2 || (jb.traversal == 0 && default_sil_lane(w.dim, jb.q.n, w.n_prims))) &&
+                              w.bin_depth + 2 <= kRayStackMax;
         const bool lane = jb.kind == kIntersect || sil_lane;
         int g = -1;
         for (size_t k = 0; k < n_groups; ++k) {
@@ -2370,7 +2414,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             const bool fits = need <= kFrontMax;
             const int budget = jobs[j].budget != 0 ? jobs[j].budget : w.packet_budget;
             if (fits && budget >= 0) {
-                P.budget = budget > 0 ? budget : default_budget((int)G.kind, G.dim, group_q);
+                P.budget = budget > 0 ? budget : default_budget((int)G.kind, G.dim, group_q, jobs[j].q.n, w.n_prims);
                 G.defer = true;
                 G.front = std::max(G.front, need);
                 const int small = w.wave_front > 0 ? w.wave_front

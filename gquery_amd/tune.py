@@ -17,6 +17,7 @@ Timing uses a sample of the caller's queries (or uniform points in the scene box
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
 import itertools
 import time
 
@@ -25,7 +26,7 @@ import torch
 
 LEAVES = (8, 16, 24, 32)
 RAY_LEAVES = (4, 6, 8)
-BUDGETS = (128, 350, 600, 1000)
+BUDGETS = (128, 350, 1000, 3000)
 TRAVERSALS = (1, 2)
 
 
@@ -148,9 +149,17 @@ def tune_jobs(jobs, leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, metho
     best = {key: (float("inf"), None) for key in groups}
     for i in range(max(len(leaves), len(ray_leaves))):
         leaf, ray_leaf = leaves[min(i, len(leaves) - 1)], ray_leaves[min(i, len(ray_leaves) - 1)]
-        scenes = {}
-        for k, (shape, _, _, _) in enumerate(jobs):
-            scenes[k] = Scene(shape.snch, shape.scene.device, leaf, ray_leaf_prims=ray_leaf)
+        t0 = time.perf_counter()
+
+        def make(shape):
+            # host threads: the engine layout build releases the GIL
+            torch.cuda.set_device(shape.scene.device)
+            return Scene(shape.snch, shape.scene.device, leaf, ray_leaf_prims=ray_leaf)
+
+        with cf.ThreadPoolExecutor(max_workers=8) as ex:
+            scenes = dict(enumerate(ex.map(make, [j[0] for j in jobs])))
+        if log:
+            log(f"tune leaf={leaf} ray_leaf={ray_leaf}: {len(scenes)} scenes in {time.perf_counter() - t0:.0f}s")
         for key, ks in groups.items():
             kind = key[1]
             for budget, trav in itertools.product(budgets, traversals):
@@ -163,6 +172,9 @@ def tune_jobs(jobs, leaves=LEAVES, budgets=BUDGETS, traversals=TRAVERSALS, metho
                     lk = {"ray_leaf_prims": ray_leaf} if kind == "ray" else {"leaf_prims": leaf}
                     best[key] = (ms, {**lk, "packet_budget": budget, "traversal": trav})
                 del b
+            if log:
+                log(f"tune leaf={leaf} ray_leaf={ray_leaf} group dim={key[0]} kind={key[1]} "
+                    f"prims~4^{key[2]}: best so far {best[key][0]:.3f} ms")
         del scenes
     out = [None] * len(jobs)
     for key, ks in sorted(groups.items()):

@@ -122,10 +122,13 @@ typedef struct gq_scene_options {
     int32_t leaf_prims;    /* max primitives per leaf of the closest-point BVH4, 1..64; 0 = default
                               (24 in 3D, 16 in 2D) */
     int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
-                              move to the wave-per-query pass; 0 = default (closest point 350 in
-                              3D / 300 in 2D, silhouettes 128 in 3D / 750 in 2D), < 0 = never */
-    int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D, packet + wave
-                              pass in 3D), 1 = packet + wave pass, 2 = lane per query */
+                              move to the wave-per-query pass; 0 = default, by scene size and
+                              queries per primitive (closest point 128-1000, 350 in 3D / 300 in 2D
+                              for 65k-330k primitives; silhouettes 128 or 3000 in 3D, 750 in 2D;
+                              smaller for launches too small to fill the GPU), < 0 = never */
+    int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D below 393,216
+                              segments, and in 3D at 24-128 queries per triangle; otherwise
+                              packet + wave pass), 1 = packet + wave pass, 2 = lane per query */
     int32_t wave_front;    /* LDS frontier entries of the first wave-per-query pass, 1..8192;
                               0 = default (closest point 1024, silhouettes 640).  A query whose
                               frontier outgrows it is finished by a second pass with the full
