@@ -2234,6 +2234,9 @@ struct Group {
     int dim;
     bool quantized;
     bool lane;                        // lane-per-query kernel (rays, lane silhouettes)
+    bool dense;                       // 3D silhouette jobs on the long default budget (no wave pass
+                                      // to speak of): their own group, so in a large batch the
+                                      // other jobs' wave pass runs beside their long packet walks
     std::vector<int> jobs;
     size_t tiles_off = 0, n_tiles = 0;   // tiles of this group's traversal kernel
     size_t dc_off = 0, def_off = 0, ovf_off = 0;
@@ -2285,10 +2288,13 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
                               (jb.traversal == 2 || (jb.traversal == 0 && default_sil_lane(w.dim, jb.q.n, w.n_prims))) &&
                               w.bin_depth + 2 <= kRayStackMax;
         const bool lane = jb.kind == kIntersect || sil_lane;
+        const bool dense = jb.kind == kSilhouette && w.dim == 3 && !lane && jb.budget == 0 && w.packet_budget == 0 &&
+                           (double)jb.q.n >= kSil3DenseRho * (double)std::max<int64_t>(w.n_prims, 1);
         int g = -1;
         for (size_t k = 0; k < n_groups; ++k) {
             const Group &G = plan.groups[k];
-            if (G.kind == jb.kind && G.dim == w.dim && G.quantized == jb.quantized && G.lane == lane) g = (int)k;
+            if (G.kind == jb.kind && G.dim == w.dim && G.quantized == jb.quantized && G.lane == lane && G.dense == dense)
+                g = (int)k;
         }
         if (g < 0) {
             if (n_groups == plan.groups.size()) plan.groups.emplace_back();
@@ -2301,6 +2307,7 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
             G.dim = w.dim;
             G.quantized = jb.quantized;
             G.lane = lane;
+            G.dense = dense;
             g = (int)n_groups++;
         }
         plan.groups[g].jobs.push_back((int)j);
@@ -2394,10 +2401,12 @@ hipError_t make_plan(const std::vector<EngineJob> &jobs, uint8_t *base, Plan &pl
         size_t nq = 0;
         int64_t group_q = 0;
         bool shared = false;
-        for (int j : G.jobs) {
-            group_q += jobs[j].q.n;
-            shared = shared || jobs[j].shared;
-        }
+        for (int j : G.jobs) shared = shared || jobs[j].shared;
+        // launch size for the small-launch budgets: the queries of every group of this kind,
+        // dimension, node format and traversal (dense and other 3D silhouette groups run together)
+        for (const Group &H : plan.groups)
+            if (H.kind == G.kind && H.dim == G.dim && H.quantized == G.quantized && H.lane == G.lane)
+                for (int j : H.jobs) group_q += jobs[j].q.n;
         // beside other launches a small group does not leave the GPU idle: budget it as a full one
         if (shared) group_q = std::max<int64_t>(group_q, kFullLaunch);
         for (int j : G.jobs) {
