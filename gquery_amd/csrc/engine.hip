@@ -53,14 +53,14 @@ constexpr int kRayStackMax = 96;  // ray stack entries per lane (768 B of privat
 constexpr int kWavePassBlocks = 2048;  // persistent grid of the full-frontier wave-per-query pass
 constexpr int kXcds = 8;            // MI355X: 8 XCDs of 32 CUs, each with its own L2
 // Packet steps per wavefront before its queries are deferred to the wave-per-query pass, by query
-// kind and dimension (measured optimum on 1M uniform queries: 3D closest point with 24-primitive
-// leaves 2.76 ms at 350 vs 2.79 at 300, 2.86 at 250 and 3.13 at 200; 2D closest point with 16
-// 2.12 ms at 300 vs 2.14 at 400 and 2.28 at 500; 3D silhouettes, with near-last appends in the wave pass,
+// kind and dimension (measured optimum on 1M uniform queries: 3D closest point on 327,680 triangles
+// with 24-primitive leaves 2.66 ms at 300 vs 2.70 at 350, 2.78 at 250 and 3.08 at 200; 2D closest
+// point on 65,536 segments with 16 1.96 ms at 300 vs 2.05 at 450 and 2.25 at 200; 3D silhouettes, with near-last appends in the wave pass,
 // 29.05 ms at 128 vs 29.07 at 64, 29.30 at 250 and 31.47 at 750: the wave pass is the better
 // place for nearly all of their work; 2D silhouettes in packet mode (65,536 segments) 3.28 ms at
 // 750 vs 3.85 at 2,000 and 8.93 at 128: a 2D query has too little work for a whole wavefront).
 // A scene or job option overrides it.
-constexpr int kPacketBudgetCp3 = 350, kPacketBudgetCp2 = 300, kPacketBudgetSil3 = 128, kPacketBudgetSil2 = 750;
+constexpr int kPacketBudgetCp3 = 300, kPacketBudgetCp2 = 300, kPacketBudgetSil3 = 128, kPacketBudgetSil2 = 750;
 // A launch with few queries leaves most of the GPU idle, and then a packet wavefront's walk -- up to
 // its budget of dependent node fetches -- is the latency of the whole call, while the wave-per-query
 // pass puts 64 lanes on each query.  So below a launch size the default budget drops: measured on
@@ -81,9 +81,12 @@ constexpr int64_t kFullLaunch = kSmallLaunchSil3;
 // primitive (denser packets share more of their walk).  Measured with one-job launches of uniform
 // queries (displaced icospheres s = 3..8, stars of 1k-1M segments; 262k, 1M and 4M queries; ms):
 //   3D closest point: s <= 5 (<= 20k triangles) 128 beats 350 by 5-25% (s5, 1M: 1.14 vs 1.21);
+//     s6 (82k) 200 (1M: 1.63 vs 1.75 at 300); s7 300, and 450 at rho >= 8 (4M: 6.83 vs 7.11);
 //     s8 (1.3M triangles) 1,000 beats 350 (1M: 6.35 vs 8.56, 4M: 14.2 vs 17.0).
-//   2D closest point: <= 16k segments 128 beats 300 (16k, 262k queries: 0.68 vs 0.93); from 262k
-//     segments with rho >= 1, 1,000 (512k segments, 1M: 6.56 vs 10.8; 1M segments, 4M: 23.7 vs 49.5).
+//   2D closest point: <= 16k segments 128 beats 300 (16k, 262k queries: 0.68 vs 0.93); 65k
+//     segments 300, and 450 at rho >= 32 (4M: 4.82 vs 4.89); 262k segments 700 (1M: 4.17 vs 6.34
+//     at 300); from 512k segments with rho >= 1, 1,000 (512k segments, 1M: 6.56 vs 10.8; 1M
+//     segments, 4M: 23.7 vs 49.5).
 //   3D silhouettes: with rho >= 128 nearly every packet finishes and budget 3,000 wins (s4, 1M:
 //     4.96 vs 11.95; s5, 4M: 26.3 vs 46.8); for 24 <= rho < 128 lane per query wins (s4 at 262k
 //     queries 2.51 vs 3.34, s5 at 1M 9.35 vs 11.98, s6 at 4M 42.9 vs 48.2); below that 128.
@@ -93,8 +96,10 @@ constexpr int64_t kFullLaunch = kSmallLaunchSil3;
 constexpr int kLongBudget = 3000;
 constexpr int kPacketBudgetSmallTree = 128;
 constexpr int kPacketBudgetLargeTree = 1000;
-constexpr int64_t kCp3SmallTree = 1 << 16, kCp3LargeTree = 1 << 19;
-constexpr int64_t kCp2SmallTree = 1 << 14, kCp2LargeTree = 1 << 18;
+constexpr int kPacketBudgetMidTree = 200, kPacketBudgetCp2Large = 700;
+constexpr int64_t kCp3SmallTree = 1 << 16, kCp3MidTree = 1 << 17, kCp3LargeTree = 1 << 19;
+constexpr int64_t kCp2SmallTree = 1 << 14, kCp2MidTree = 1 << 17, kCp2LargeTree = 1 << 19;
+constexpr double kCp3DenseRho = 8.0, kCp2DenseRho = 32.0;
 constexpr int64_t kSil2PacketTree = 3 << 17;
 constexpr double kSil3LaneRho = 24.0, kSil3DenseRho = 128.0;
 
@@ -112,12 +117,16 @@ int default_budget(int kind, int dim, int64_t group_q, int64_t n, int64_t prims)
     }
     if (dim == 3) {
         if (group_q < kSmallLaunchCp3) return kSmallBudgetCp;
-        return prims < kCp3SmallTree ? kPacketBudgetSmallTree
-             : prims < kCp3LargeTree ? kPacketBudgetCp3 : kPacketBudgetLargeTree;
+        if (prims < kCp3SmallTree) return kPacketBudgetSmallTree;
+        if (prims < kCp3MidTree) return kPacketBudgetMidTree;
+        if (prims < kCp3LargeTree) return rho >= kCp3DenseRho ? kPacketBudgetCp3 * 3 / 2 : kPacketBudgetCp3;
+        return kPacketBudgetLargeTree;
     }
     if (group_q < kSmallLaunchCp2) return kSmallBudgetCp;
     if (prims <= kCp2SmallTree) return kPacketBudgetSmallTree;
-    return prims >= kCp2LargeTree && rho >= 1.0 ? kPacketBudgetLargeTree : kPacketBudgetCp2;
+    if (prims < kCp2MidTree) return rho >= kCp2DenseRho ? kPacketBudgetCp2 * 3 / 2 : kPacketBudgetCp2;
+    if (prims < kCp2LargeTree) return kPacketBudgetCp2Large;
+    return rho >= 1.0 ? kPacketBudgetLargeTree : kPacketBudgetCp2;
 }
 
 // Default silhouette traversal of a job (no job or scene override): lane per query for 2D scenes
@@ -950,6 +959,19 @@ __device__ __forceinline__ float wave_min(float v) {
 }
 
 
+// Exclusive prefix sum of v over the wavefront's lanes; total receives the sum over all 64.
+__device__ __forceinline__ int wave_excl_scan(int v, int &total) {
+    const int lane = threadIdx.x & 63;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
 // Appends one query (index, bound, best slot, job) to a hand-off list: every lane takes part in the
 // atomic (lane 0 adds 1) and stores the same entry, so no lane-dependent branch is involved.
 __device__ __forceinline__ void wave_handoff(uint4 *list, uint32_t *count, uint4 entry) {
@@ -1016,6 +1038,7 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
             bool hit[4] = {false, false, false, false};
             float cand = kInf;
             int cand_slot = -1;
+            uint32_t lfirst = 0, lcount = 0;
             if (lane < take && __uint_as_float(e.y) <= bound2) {
                 if (!ref_is_leaf(e.x)) {
                     NodeC4<D> nd;
@@ -1031,27 +1054,43 @@ __device__ __forceinline__ void cp_wave_item(const GP &P, const uint4 &item, uin
                         ent[j] = make_uint2(nd.child[j], __float_as_uint(lb2));
                     }
                 } else {
-                    uint32_t first, count;
-                    leaf_range(e.x, (const uint32_t *)P.big, first, count);
-                    for (uint32_t s = first; s < first + count; ++s) {
-                        float dd;
-                        if constexpr (D == 3) {
-                            const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
-                            const float4 pl = ldg(P.prims + (4 * s + 3));
-                            if (!tri_may_beat(pc, a, b, c, pl, bound2)) continue;
-                            V3 pt;
-                            V2 uv;
-                            dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
-                        } else {
-                            const float4 ab = ldg(P.prims + (s));
-                            V2 pt;
-                            dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
-                        }
-                        if (dd < cand || (dd == cand && (int)s < cand_slot)) {
-                            cand = dd;
-                            cand_slot = (int)s;
-                        }
-                    }
+                    leaf_range(e.x, (const uint32_t *)P.big, lfirst, lcount);
+                }
+            }
+            // the round's leaf primitives, one per lane: their slots are listed in the free part of
+            // the frontier (entries [size, cap) were taken into registers above), so a leaf's
+            // primitives are tested side by side instead of one after another by its lane
+            auto test = [&](uint32_t s) {
+                float dd;
+                if constexpr (D == 3) {
+                    const float4 a = ldg(P.prims + (4 * s)), b = ldg(P.prims + (4 * s + 1)), c = ldg(P.prims + (4 * s + 2));
+                    const float4 pl = ldg(P.prims + (4 * s + 3));
+                    if (!tri_may_beat(pc, a, b, c, pl, bound2)) return;
+                    V3 pt;
+                    V2 uv;
+                    dd = closest_point_triangle(p, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), pt, uv);
+                } else {
+                    const float4 ab = ldg(P.prims + (s));
+                    V2 pt;
+                    dd = closest_point_segment(p, v2(ab.x, ab.y), v2(ab.z, ab.w), pt);
+                }
+                if (dd < cand || (dd == cand && (int)s < cand_slot)) {
+                    cand = dd;
+                    cand_slot = (int)s;
+                }
+            };
+            if (__ballot(lcount > 0u) != 0ull) {
+                int ltot;
+                const int loff = wave_excl_scan((int)lcount, ltot);
+                if (ltot <= 2 * (cap - size)) {
+                    uint32_t *list = reinterpret_cast<uint32_t *>(front + size);
+                    for (uint32_t t = 0; t < lcount; ++t) list[loff + (int)t] = lfirst + t;
+                    wave_sync_lds();
+                    for (int b0 = 0; b0 < ltot; b0 += 64)
+                        if (b0 + lane < ltot) test(list[b0 + lane]);
+                    wave_sync_lds();   // the appends below reuse this space
+                } else {
+                    for (uint32_t s = lfirst; s < lfirst + lcount; ++s) test(s);
                 }
             }
             // wave minimum of the leaf candidates (ties to the lower slot, so every lane agrees)

@@ -123,9 +123,9 @@ typedef struct gq_scene_options {
                               (24 in 3D, 16 in 2D) */
     int32_t packet_budget; /* traversal steps a wavefront of 64 queries may take before its queries
                               move to the wave-per-query pass; 0 = default, by scene size and
-                              queries per primitive (closest point 128-1000, 350 in 3D / 300 in 2D
-                              for 65k-330k primitives; silhouettes 128 or 3000 in 3D, 750 in 2D;
-                              smaller for launches too small to fill the GPU), < 0 = never */
+                              queries per primitive (closest point 128-1000, 300 for 65k-330k
+                              primitives; silhouettes 128 or 3000 in 3D, 750 in 2D; smaller for
+                              launches too small to fill the GPU), < 0 = never */
     int32_t traversal;     /* silhouette queries: 0 = auto (lane per query in 2D below 393,216
                               segments, and in 3D at 24-128 queries per triangle; otherwise
                               packet + wave pass), 1 = packet + wave pass, 2 = lane per query */
