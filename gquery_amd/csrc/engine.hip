@@ -68,7 +68,10 @@ constexpr int kPacketBudgetCp3 = 300, kPacketBudgetCp2 = 300, kPacketBudgetSil3 
 // vs the default): 300 queries 0.57 vs 1.18, 16,384 0.72 vs 1.44, 65,536 1.40 vs 1.70, 262,144
 // 3.60 vs 1.79; 65,536-segment star with 16-primitive leaves: 16,384 0.45 vs 1.05, 65,536 0.99 vs
 // 1.08, 262,144 3.32 vs 1.38; 3D silhouettes at budget 1: 300 0.42 vs 0.25, 65,536 1.53 vs 1.36.
-constexpr int kSmallBudgetCp = 16, kSmallBudgetSil3 = 1;
+// Below 16,384 queries 8 steps do better still (300 queries 0.32 vs 0.37 ms, 8,000 0.57 vs 0.60;
+// 32,768: 1.00 vs 0.79, so 16 from there).
+constexpr int kSmallBudgetCp = 16, kTinyBudgetCp = 8, kSmallBudgetSil3 = 1;
+constexpr int64_t kTinyLaunchCp = 1 << 14;
 constexpr int64_t kSmallLaunchCp3 = 1 << 17, kSmallLaunchCp2 = 1 << 17, kSmallLaunchSil3 = 1 << 18;
 // When other launches share the device (gq_job.shared; a serving loop's stream slots) the GPU is not
 // idle and total work decides throughput and queueing: shared groups are budgeted as full launches
@@ -115,6 +118,7 @@ int default_budget(int kind, int dim, int64_t group_q, int64_t n, int64_t prims)
         if (prims >= kSil2PacketTree) return rho >= 1.0 ? kLongBudget : kPacketBudgetSmallTree;
         return kPacketBudgetSil2;
     }
+    if (group_q < kTinyLaunchCp) return kTinyBudgetCp;
     if (dim == 3) {
         if (group_q < kSmallLaunchCp3) return kSmallBudgetCp;
         if (prims < kCp3SmallTree) return kPacketBudgetSmallTree;
