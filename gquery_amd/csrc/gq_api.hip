@@ -263,7 +263,7 @@ int gq_scene_create(const gq_structure *st, gq_scene **out) { return gq_scene_cr
 int gq_scene_create_ex(const gq_structure *st, const gq_scene_options *opt, gq_scene **out) {
     if (!st || !out) return fail(GQ_ERR_INVALID, "null argument");
     const int leaf_prims = (opt && opt->leaf_prims > 0) ? opt->leaf_prims : default_cp_leaf(st->h.dim);
-    const int ray_leaf = (opt && opt->ray_leaf_prims > 0) ? opt->ray_leaf_prims : kDefaultRayLeaf;
+    const int ray_leaf = (opt && opt->ray_leaf_prims > 0) ? opt->ray_leaf_prims : default_ray_leaf(st->h.dim, st->h.n_prims);
     if (leaf_prims > kMaxLeafOption || ray_leaf > kMaxLeafOption)
         return fail(GQ_ERR_INVALID, "leaf_prims and ray_leaf_prims must be in 1..64");
     if (opt && (opt->traversal < 0 || opt->traversal > 2)) return fail(GQ_ERR_INVALID, "traversal must be 0, 1 or 2");
@@ -517,7 +517,7 @@ int gq_structure_engine_layout(const gq_structure *st, int which, void *out, int
     if (!st || !n_bytes) return fail(GQ_ERR_INVALID, "null argument");
     WideHost wh;
     try {
-        build_wide(st->h, wh, default_cp_leaf(st->h.dim), kDefaultRayLeaf);
+        build_wide(st->h, wh, default_cp_leaf(st->h.dim), default_ray_leaf(st->h.dim, st->h.n_prims));
     } catch (const std::bad_alloc &) {
         return fail(GQ_ERR_OOM, "host allocation failed");
     }

@@ -28,8 +28,19 @@ struct RefCone;
 // make large leaves pay; 1M uniform queries, 3D 327,680 triangles: leaf 6 3.47 ms, 16 2.91, 24
 // 2.76, 32 2.79, 48 3.07; 2D 65,536 segments: 6 2.30, 16 2.12, 24 2.20) and the ray tree (a lane
 // tests its leaf serially: 3D 0.49 ms at 6, 0.63 at 16, 0.75 at 24).
-constexpr int kDefaultCpLeaf3 = 24, kDefaultCpLeaf2 = 16, kDefaultRayLeaf = 6;
+constexpr int kDefaultCpLeaf3 = 24, kDefaultCpLeaf2 = 16;
 inline int default_cp_leaf(int dim) { return dim == 3 ? kDefaultCpLeaf3 : kDefaultCpLeaf2; }
+// Rays test a leaf's primitives one after another in their lane, so their tree wants small leaves
+// (1M uniform rays, ms: 327,680 triangles 0.463 at 2, 0.487 at 4, 0.498 at 6, 0.522 at 8; 20,480
+// triangles 0.356 at 2, 0.366 at 6; 524,288-segment star 0.435 at 2 or 3, 0.501 at 4, 0.535 at 6),
+// except a mid-size polyline, whose segment tests are cheaper than a node step (65,536-segment
+// star 0.348 at 6, 0.350 at 4, 0.401 at 3, 0.442 at 2).
+constexpr int kDefaultRayLeaf3 = 2, kDefaultRayLeaf2 = 6, kDefaultRayLeaf2Large = 3;
+constexpr int64_t kRayLeaf2LargeScene = int64_t(1) << 18;
+inline int default_ray_leaf(int dim, int64_t n_prims) {
+    if (dim == 3) return kDefaultRayLeaf3;
+    return n_prims >= kRayLeaf2LargeScene ? kDefaultRayLeaf2Large : kDefaultRayLeaf2;
+}
 constexpr int kMaxLeafOption = 64;   // largest gq_scene_options.leaf_prims / ray_leaf_prims
 
 struct WideView {

@@ -2,7 +2,7 @@
 
 Knobs (all exact: every candidate returns the same records):
   * leaf_prims     -- primitives per leaf of the closest-point BVH4 (8, 16, 24 or 32);
-  * ray_leaf_prims -- primitives per leaf of the rays' BVH4 (4, 6 or 8);
+  * ray_leaf_prims -- primitives per leaf of the rays' BVH4 (2, 3, 4 or 6);
   * packet_budget  -- packet steps a wavefront takes before its queries move to the
                       wave-per-query pass (how much work runs wave-per-query vs 64-lane packets);
   * traversal      -- silhouettes: packet + wave-per-query pass, or lane per query.
@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 LEAVES = (8, 16, 24, 32)
-RAY_LEAVES = (4, 6, 8)
+RAY_LEAVES = (2, 3, 4, 6)
 BUDGETS = (128, 350, 1000, 3000)
 TRAVERSALS = (1, 2)
 

@@ -138,7 +138,8 @@ typedef struct gq_scene_options {
                               1 = quantized (64 B per pair: 8-bit boxes in 3D, 16-bit in 2D, int16
                               cones, 2D reference cones from a table).  Results do not change */
     int32_t ray_leaf_prims; /* max primitives per leaf of the rays' BVH4 (the same binary tree
-                              collapsed separately), 1..64; 0 = default (6) */
+                              collapsed separately), 1..64; 0 = default (2 in 3D; in 2D 6,
+                              or 3 from 262,144 segments) */
     int32_t reserved[2];
 } gq_scene_options;
 int gq_scene_create_ex(const gq_structure *s, const gq_scene_options *opt, gq_scene **out);
